@@ -1,0 +1,243 @@
+"""Throughput of the fused stream-collide step on one MI355X (prints one JSON line).
+
+    python bench.py [--steps 200] [--warmup 20] [--quick] [--profile] [--no-cpu]
+
+Workload of the headline number: D2Q9, fully periodic 4096 x 4096, fp32 populations, BGK
+tau = 0.6, two-buffer fused pull kernel (hip/fused), initial state the equilibrium of the
+shear wave u_x = 0.01 sin(2 pi y / 4096), u_y = 0, rho = 1 + 1e-3 U(-1, 1) per cell from a
+seeded generator.  Populations are resident in HBM before timing; ``steps`` launches are
+timed with HIP events on the launch stream.  The same line carries every other variant of
+that configuration (TRT, MRT, AA, fp16/bf16 shifted storage), the D3Q19 512^3 TRT
+Taylor-Green runs (fp32, fp16, bf16; pull and AA), the measured copy ceiling, the roofline
+of the dominant kernel and the float64 NumPy oracle timed on the host (CPU baseline).
+
+``--profile`` runs only the headline kernel and the copy kernel (for rocprofv3).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import statistics
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "MLUPS + % HBM roofline (D2Q9 4096², D3Q19 512³); fwd+adjoint MLUPS"
+SEED = 20261203
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def shear_wave_init(n: int, seed: int):
+    rng = np.random.default_rng(seed)
+    rho = 1.0 + 1e-3 * rng.uniform(-1.0, 1.0, size=(n, n))
+    u = np.zeros((2, n, n))
+    u[0] = 0.01 * np.sin(2 * np.pi * np.arange(n) / n)[:, None]
+    return rho, u
+
+
+def tgv_init_device(n: int, amp: float = 0.02):
+    import torch
+
+    k = torch.arange(n, dtype=torch.float64, device="cuda") * (2 * np.pi / n)
+    s, c = torch.sin(k), torch.cos(k)
+    u = torch.zeros((3, n, n, n), dtype=torch.float64, device="cuda")  # (a, z, y, x)
+    u[0] = amp * s[None, None, :] * c[None, :, None] * c[:, None, None]
+    u[1] = -amp * c[None, None, :] * s[None, :, None] * c[:, None, None]
+    return None, u
+
+
+def problem_2d(coll: str, n: int, rho, u):
+    from pylbm_amd.problem import Problem
+
+    extra = {"s_e": 1.2, "s_eps": 1.2} if coll == "mrt" else {}
+    return Problem((n, n), tau=0.6, collision=coll, periodic=(True, True), **extra)
+
+
+def run_variant(variant: str, problem, init, steps: int, warmup: int, repeats: int = 3):
+    import torch
+
+    from pylbm_amd import backends as B
+    from pylbm_amd.benchmark import time_steps
+
+    s = B.make("hip", variant, problem, init="none")
+    s.init_equilibrium(*init)
+    t = time_steps(s, steps, warmup, repeats)
+    ev = statistics.median(t["event_s"])
+    n = problem.n_nodes
+    rec = {
+        "variant": variant, "collision": problem.collision, "lattice": problem.lattice.name,
+        "size": list(problem.shape), "storage": s.storage, "steps": t["steps"],
+        "MLUPS": n * t["steps"] / ev / 1e6,
+        "MLUPS_wall": n * t["steps"] / statistics.median(t["wall_s"]) / 1e6,
+        "us_per_step": ev / t["steps"] * 1e6, "bytes_per_update": s.bytes_per_node,
+    }
+    rec["GBs"] = rec["MLUPS"] * 1e6 * s.bytes_per_node / 1e9
+    ok = np.isfinite(s.reduce("mass"))
+    rec["finite"] = bool(ok)
+    del s
+    torch.cuda.empty_cache()
+    return rec
+
+
+def cpu_baseline(n: int, steps: int) -> dict:
+    """The float64 NumPy roll oracle on the host, D2Q9 periodic shear wave n^2."""
+    from oracle import lbm_ref as R
+
+    rho, u = shear_wave_init(n, SEED)
+    case = R.Case(shape=(n, n), tau=0.6, periodic=(True, True), init_rho=rho, init_u=u)
+    f = R.initial_populations(case)
+    f = R.step(f, case)  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        f = R.step(f, case)
+    dt = time.perf_counter() - t0
+    return {"value": n * n * steps / dt / 1e6, "unit": "MLUPS", "cores": 1, "kind": "port",
+            "sample": f"D2Q9 {n}x{n} periodic shear wave, BGK tau=0.6, {steps} steps after 1 "
+                      f"warm-up step, float64 NumPy roll oracle (oracle/lbm_ref.py), "
+                      f"single-threaded; host has {os.cpu_count()} logical CPUs "
+                      f"({platform.processor() or platform.machine()})",
+            "seconds": dt}
+
+
+def load_traffic() -> dict | None:
+    p = ROOT / "profiles" / "traffic_d2q9_4096_f32_bgk.json"
+    if p.exists():
+        return json.loads(p.read_text())
+    return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--quick", action="store_true", help="headline + copy ceiling only")
+    ap.add_argument("--profile", action="store_true",
+                    help="only the headline kernel and the copy kernel (for rocprofv3)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--n2", type=int, default=4096)
+    ap.add_argument("--n3", type=int, default=512)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported by this build "
+              "(multi-GPU slab decomposition is design-only).", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from pylbm_amd import _lib
+    from pylbm_amd.benchmark import SPEC_BANDWIDTH_GBS, copy_bandwidth
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP GPU", file=sys.stderr)
+        return 1
+    _lib.load()
+    dev = torch.cuda.get_device_name(0)
+    log(f"device {dev}")
+    n = a.n2
+    rho, u = shear_wave_init(n, SEED)
+
+    if a.profile:
+        from pylbm_amd import backends as B
+
+        p = problem_2d("bgk", n, rho, u)
+        s = B.make("hip", "fused", p, init="none")
+        s.init_equilibrium(rho, u)
+        s.step(a.steps)
+        s.synchronize()
+        log(f"profile: {a.steps} steps of hip/fused D2Q9 {n}^2 done")
+        nbytes = 2 * 9 * n * n * 4 // 2  # one plane set: the bytes one step reads
+        x = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        y = torch.empty_like(x)
+        st = int(torch.cuda.current_stream().cuda_stream)
+        for _ in range(a.steps):
+            _lib.call("pylbm_hip_copy", x.data_ptr(), y.data_ptr(), nbytes, st)
+        torch.cuda.synchronize()
+        log(f"profile: {a.steps} copies of {nbytes} B done")
+        return 0
+
+    log("copy ceiling (4 GiB buffers)")
+    ceil = copy_bandwidth(4096 if not a.quick else 1024)
+    cbw = ceil["copy_bandwidth_GBs"]
+    log(f"copy ceiling {cbw:.0f} GB/s (hip copy {ceil['hip_copy_GBs']:.0f}, "
+        f"DtoD {ceil['memcpy_dtod_GBs']:.0f})")
+
+    head = run_variant("fused", problem_2d("bgk", n, rho, u), (rho, u), a.steps, a.warmup)
+    log(f"headline hip/fused BGK {n}^2: {head['MLUPS']:.0f} MLUPS, {head['GBs']:.0f} GB/s")
+    variants = [head]
+    if not a.quick:
+        for coll in ("bgk", "trt", "mrt"):
+            for v in ("fused", "fused-aa", "fused-fp16", "fused-bf16", "fused-aa-fp16",
+                      "fused-aa-bf16"):
+                if coll == "bgk" and v == "fused":
+                    continue
+                r = run_variant(v, problem_2d(coll, n, rho, u), (rho, u), a.steps, a.warmup)
+                variants.append(r)
+                log(f"  {coll} hip/{v}: {r['MLUPS']:.0f} MLUPS {r['GBs']:.0f} GB/s")
+        from pylbm_amd.problem import Problem
+
+        n3 = a.n3
+        p3 = Problem((n3, n3, n3), tau=0.55, collision="trt", periodic=(True, True, True))
+        init3 = tgv_init_device(n3)
+        for v in ("fused", "fused-fp16", "fused-bf16", "fused-aa", "fused-aa-fp16",
+                  "fused-aa-bf16"):
+            r = run_variant(v, p3, init3, 100, 10, repeats=2)
+            variants.append(r)
+            log(f"  D3Q19 {n3}^3 trt hip/{v}: {r['MLUPS']:.0f} MLUPS {r['GBs']:.0f} GB/s")
+        del init3
+        torch.cuda.empty_cache()
+    for r in variants:
+        r["pct_copy"] = 100.0 * r["GBs"] / cbw
+        r["pct_spec"] = 100.0 * r["GBs"] / SPEC_BANDWIDTH_GBS
+
+    t_launch = head["us_per_step"] * 1e-6
+    achieved = head["bytes_per_update"] * n * n / t_launch / 1e9
+    traffic = load_traffic()
+    roof = {"bound": "hbm", "achieved": achieved, "peak": SPEC_BANDWIDTH_GBS, "unit": "GB/s",
+            "frac": achieved / SPEC_BANDWIDTH_GBS,
+            "traffic": traffic["bytes_per_launch"] if traffic else None,
+            "kernel": "k_step<D2Q9,BGK,F32,AB,uniform-force>",
+            "algorithmic_bytes_per_launch": head["bytes_per_update"] * n * n,
+            "avg_launch_us": t_launch * 1e6,
+            "frac_of_measured_copy": achieved / cbw}
+
+    cpu = None
+    if not a.no_cpu and not a.quick:
+        log(f"CPU baseline: NumPy float64 oracle, D2Q9 {n}^2, {a.cpu_steps} steps")
+        cpu = cpu_baseline(n, a.cpu_steps)
+        log(f"CPU baseline {cpu['value']:.2f} MLUPS ({cpu['seconds']:.1f} s)")
+
+    line = {
+        "metric": METRIC, "value": head["MLUPS"], "unit": "MLUPS", "n_gpus": 1,
+        "steps": head["steps"], "warmup": a.warmup, "ms_per_step": head["us_per_step"] / 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": f"synthetic (seeded shear wave, seed {SEED})",
+        "config": {"workload": f"D2Q9 fused pull (hip/fused), periodic {n}x{n} shear wave, "
+                               "fp32 populations, BGK tau=0.6",
+                   "lattice": "D2Q9", "size": [n, n], "collision": "bgk", "tau": 0.6,
+                   "storage": "f32", "pattern": "ab", "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "copy_ceiling": ceil,
+        "pct_copy": head["pct_copy"],
+        "variants": variants,
+        "device": dev,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,142 @@
+/*
+ * pylbm_hip.h -- C ABI of libpylbm_hip.so, the gfx950 (MI355X) fused lattice-Boltzmann
+ * stream-collide library behind pylbm's `hip` backend.
+ *
+ * Conventions (identical to the reference so the canonical state is interchangeable):
+ *   - populations are SoA planes f[i * N + n], n = x + nx * (y + ny * z), x contiguous;
+ *     N = nx * ny * nz; plane offsets are 64-bit, in-plane indices 32-bit (N < 2^31);
+ *   - D2Q9 order: 0 rest, E, N, W, S, NE, NW, SW, SE; D3Q19 opposite pairs adjacent
+ *     (pylbm/lattice.py:49-67);
+ *   - storage PYLBM_F32 keeps f; PYLBM_F16 / PYLBM_BF16 keep h = f - w_i (arithmetic fp32);
+ *   - AA layout (params.aa = 1): the canonical post-collision population i sits in slot
+ *     opp(i) (pylbm/backends/_fused.py:83-103);
+ *   - the moving lid is the top wall of the last axis, moving in +x (Ladd term).
+ *
+ * Every entry point returns 0 on success or a nonzero code (a hipError_t, or one of the
+ * PYLBM_E* codes below); pylbm_hip_last_error() returns the message of the last failure
+ * on the calling thread.  `stream` is a hipStream_t passed as void* (NULL = null stream).
+ * All pointers are device pointers unless stated otherwise.  Launches are asynchronous.
+ */
+#ifndef PYLBM_HIP_H
+#define PYLBM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PYLBM_HIP_ABI_VERSION 1
+
+enum { PYLBM_D2Q9 = 0, PYLBM_D3Q19 = 1 };
+enum { PYLBM_BGK = 0, PYLBM_TRT = 1, PYLBM_MRT = 2 };
+enum { PYLBM_F32 = 0, PYLBM_F16 = 1, PYLBM_BF16 = 2 };
+enum { PYLBM_AB = 0, PYLBM_AA_ODD = 1, PYLBM_AA_EVEN = 2 };
+enum { PYLBM_RED_MASS = 0, PYLBM_RED_KINETIC = 1, PYLBM_RED_MAXABS_DIFF = 2 };
+
+enum {
+  PYLBM_EINVAL = 1001,   /* bad parameters (message says which) */
+  PYLBM_ERANGE = 1002,   /* size outside what the kernels index */
+  PYLBM_ENOTSUP = 1003   /* combination not implemented (e.g. MRT on D3Q19) */
+};
+
+/*
+ * Kernel parameters of one box problem.  Mirrors the compile-time KernelSpec and runtime
+ * scalars of the reference (pylbm/codegen.py:36-51, pylbm/backends/_fused.py:129-136,
+ * pylbm/backends/cupy_backend.py:51-70); the runtime rates follow runtime_rates
+ * (pylbm/codegen.py:437-445): BGK r0 = 1/tau; TRT (r0, r1) = (s+, s-);
+ * MRT (r0, r1, r2, r3) = (s_nu, s_e, s_eps, s_q).
+ */
+typedef struct pylbm_params {
+  int32_t lattice;      /* PYLBM_D2Q9 | PYLBM_D3Q19 */
+  int32_t collision;    /* PYLBM_BGK | PYLBM_TRT | PYLBM_MRT (D2Q9 only) */
+  int32_t storage;      /* PYLBM_F32 | PYLBM_F16 | PYLBM_BF16 */
+  int32_t aa;           /* 1: AA layout (slot opp(i) holds population i), else 0 */
+  int32_t periodic[3];  /* per axis x, y, z: 1 periodic, 0 half-way bounce-back walls */
+  int32_t nx, ny, nz;   /* nz = 1 for D2Q9 */
+  int64_t n_nodes;      /* N = nx * ny * nz (checked) */
+  float ulid;           /* lid speed (top wall of the last axis), 0 = none */
+  float force[3];       /* uniform Guo body force (ignored when force_field != NULL) */
+  float rates[4];       /* r0..r3, see above */
+  const float* force_field; /* NULL, or per-cell Guo force, SoA (d, N) fp32, device */
+} pylbm_params;
+
+/* ABI version (PYLBM_HIP_ABI_VERSION of the library). */
+int pylbm_hip_abi_version(void);
+
+/* Message of the last nonzero return on this thread ("" if none). Host pointer. */
+const char* pylbm_hip_last_error(void);
+
+/*
+ * One fused time step: pull streaming + bounce-back + Ladd lid, moments, equilibrium,
+ * Guo forcing, BGK/TRT/MRT collision, store.  pattern = PYLBM_AB reads src writes dst
+ * (src != dst); PYLBM_AA_ODD / PYLBM_AA_EVEN work in place (src == dst required).
+ * Replaces FusedSolver._launch (pylbm/backends/_fused.py:108-110,
+ * pylbm/backends/cupy_backend.py:101-103) and the kernel lbm_kernel
+ * (pylbm/backends/cuda/lbm.cu:50-53).
+ */
+int pylbm_hip_step(const pylbm_params* p, const void* src, void* dst, int pattern,
+                   void* stream);
+
+/*
+ * nsteps time steps enqueued from C.  Two-buffer (p->aa == 0): ping-pong a -> b -> a ...;
+ * after an odd nsteps the state is in b.  AA (p->aa == 1): nsteps must be even, b is
+ * ignored, pairs of odd/even steps in place in a.
+ * Replaces FusedSolver._advance (pylbm/backends/_fused.py:116-124) and Solver.step
+ * (pylbm/backends/base.py:68-73).
+ */
+int pylbm_hip_run(const pylbm_params* p, void* a, void* b, int64_t nsteps, void* stream);
+
+/*
+ * Canonical float64 state (q, N) -> stored layout (shift by -w_i for 16-bit storage,
+ * rounded float64 -> float32 -> storage; AA slot swap).
+ * Replaces FusedSolver._encode / set_populations (pylbm/backends/_fused.py:83-93).
+ */
+int pylbm_hip_encode(const pylbm_params* p, const double* canon, void* f, void* stream);
+
+/*
+ * Stored layout -> canonical float64 state (q, N) (exact widening, + w_i, AA unswap).
+ * Replaces FusedSolver.populations (pylbm/backends/_fused.py:95-103).
+ */
+int pylbm_hip_decode(const pylbm_params* p, const void* f, double* canon, void* stream);
+
+/*
+ * Equilibrium initial state written in the stored layout, computed in float64 then
+ * encoded exactly as pylbm_hip_encode.  rho: N values or NULL (rho = 1); u: (d, N) values
+ * or NULL (u = 0).  Replaces Problem.initial_populations + _encode
+ * (pylbm/problem.py:108-115, pylbm/lattice.py:112-119) without a host round trip.
+ */
+int pylbm_hip_init_equilibrium(const pylbm_params* p, const double* rho, const double* u,
+                               void* f, void* stream);
+
+/*
+ * Post-collision observables in float64: rho (N) and u (d, N) with u = (sum c f - F/2)/rho.
+ * Either output may be NULL.  Replaces Solver.macroscopic -> core.macroscopic
+ * (pylbm/backends/base.py:79-83, pylbm/core.py:255-266).
+ */
+int pylbm_hip_macroscopic(const pylbm_params* p, const void* f, double* rho, double* u,
+                          void* stream);
+
+/*
+ * Global float64 reductions of a stored state into *out (device pointer, one double):
+ *   PYLBM_RED_MASS        sum over nodes and populations of f
+ *   PYLBM_RED_KINETIC     sum over nodes of 0.5 rho |u|^2 (post-collision u)
+ *   PYLBM_RED_MAXABS_DIFF max |f - g| over all canonical values, other = second state
+ *                         with the same params.
+ * No reference counterpart as a kernel: the reference computes these on the host from
+ * populations() (pylbm/validation/taylor_green.py:43-44, tests/test_reference.py:10-15).
+ */
+int pylbm_hip_reduce(const pylbm_params* p, int kind, const void* f, const void* other,
+                     double* out, void* stream);
+
+/*
+ * Streaming copy of nbytes (multiple of 16) from src to dst: the measured copy ceiling.
+ * Replaces the torch copy_ bandwidth probe (bench/roofline.py:30-51).
+ */
+int pylbm_hip_copy(const void* src, void* dst, int64_t nbytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYLBM_HIP_H */

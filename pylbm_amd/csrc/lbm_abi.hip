@@ -1,0 +1,245 @@
+// C ABI of libpylbm_hip.so (declared in include/pylbm_hip.h): parameter checks, kernel
+// argument setup, the C-side step loop and error reporting.  Device memory is owned by the
+// caller (in pylbm's hip backend: torch tensors); nothing here allocates.
+#include <stdarg.h>
+#include <stdio.h>
+
+#include <string>
+
+#include "lbm_aux.hpp"
+#include "lbm_step.hpp"
+
+namespace plbm {
+template <class L>
+int launch_step(const KArgs& a, int coll, int storage, int pattern, bool field,
+                const void* src, void* dst, hipStream_t s);
+extern template int launch_step<D2Q9>(const KArgs&, int, int, int, bool, const void*, void*,
+                                      hipStream_t);
+extern template int launch_step<D3Q19>(const KArgs&, int, int, int, bool, const void*, void*,
+                                       hipStream_t);
+}  // namespace plbm
+
+using namespace plbm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+int hip_fail(int code, const char* what) {
+  if (code == 0) return 0;
+  if (code >= PYLBM_EINVAL && code <= PYLBM_ENOTSUP)
+    return fail(code, "%s: unsupported parameter combination", what);
+  return fail(code, "%s: HIP error %d (%s)", what, code,
+              hipGetErrorString((hipError_t)code));
+}
+
+int check_params(const pylbm_params* p, const char* what) {
+  if (!p) return fail(PYLBM_EINVAL, "%s: params is NULL", what);
+  if (p->lattice != PYLBM_D2Q9 && p->lattice != PYLBM_D3Q19)
+    return fail(PYLBM_EINVAL, "%s: unknown lattice %d", what, p->lattice);
+  if (p->collision < PYLBM_BGK || p->collision > PYLBM_MRT)
+    return fail(PYLBM_EINVAL, "%s: unknown collision %d", what, p->collision);
+  if (p->collision == PYLBM_MRT && p->lattice != PYLBM_D2Q9)
+    return fail(PYLBM_ENOTSUP, "%s: MRT is implemented for D2Q9 only", what);
+  if (p->storage < PYLBM_F32 || p->storage > PYLBM_BF16)
+    return fail(PYLBM_EINVAL, "%s: unknown storage %d", what, p->storage);
+  if (p->aa != 0 && p->aa != 1) return fail(PYLBM_EINVAL, "%s: aa must be 0 or 1", what);
+  if (p->nx < 1 || p->ny < 1 || p->nz < 1)
+    return fail(PYLBM_EINVAL, "%s: sizes must be >= 1 (got %d x %d x %d)", what, p->nx,
+                p->ny, p->nz);
+  if (p->lattice == PYLBM_D2Q9 && p->nz != 1)
+    return fail(PYLBM_EINVAL, "%s: D2Q9 needs nz == 1", what);
+  const int64_t N = (int64_t)p->nx * p->ny * p->nz;
+  if (N != p->n_nodes)
+    return fail(PYLBM_EINVAL, "%s: n_nodes %lld != nx*ny*nz = %lld", what,
+                (long long)p->n_nodes, (long long)N);
+  if (N >= ((int64_t)1 << 31) - kBlock)
+    return fail(PYLBM_ERANGE, "%s: %lld nodes exceed the 32-bit in-plane index", what,
+                (long long)N);
+  const int last = p->lattice == PYLBM_D2Q9 ? 1 : 2;
+  if (p->ulid != 0.0f && p->periodic[last])
+    return fail(PYLBM_EINVAL, "%s: a moving lid needs a wall on the last axis", what);
+  if (p->lattice == PYLBM_D2Q9 && p->force_field == nullptr && p->force[2] != 0.0f)
+    return fail(PYLBM_EINVAL, "%s: D2Q9 has no z force", what);
+  return 0;
+}
+
+int make_kargs(const pylbm_params* p, KArgs& a, const char* what) {
+  a.nx = p->nx;
+  a.ny = p->ny;
+  a.nz = p->nz;
+  a.N = (int)p->n_nodes;
+  a.nseg = (p->nx + kBlock - 1) / kBlock;
+  const int64_t nblk = (int64_t)a.nseg * p->ny * p->nz;
+  if (nblk >= ((int64_t)1 << 31)) return fail(PYLBM_ERANGE, "%s: grid too large", what);
+  a.nblk = (int)nblk;
+  const int dims = p->lattice == PYLBM_D2Q9 ? 2 : 3;
+  a.per = 0;
+  for (int k = 0; k < dims; ++k)
+    if (p->periodic[k]) a.per |= 1 << k;
+  a.remap = (a.nblk % 8 == 0) ? 1 : 0;
+  a.ulid = p->ulid;
+  a.fx = p->force[0];
+  a.fy = p->force[1];
+  a.fz = p->lattice == PYLBM_D2Q9 ? 0.0f : p->force[2];
+  a.r0 = p->rates[0];
+  a.r1 = p->rates[1];
+  a.r2 = p->rates[2];
+  a.r3 = p->rates[3];
+  a.ff = p->force_field;
+  return 0;
+}
+
+int step_impl(const pylbm_params* p, const KArgs& a, const void* src, void* dst, int pattern,
+              hipStream_t s) {
+  const bool field = p->force_field != nullptr;
+  if (p->lattice == PYLBM_D2Q9)
+    return launch_step<D2Q9>(a, p->collision, p->storage, pattern, field, src, dst, s);
+  return launch_step<D3Q19>(a, p->collision, p->storage, pattern, field, src, dst, s);
+}
+
+AuxArgs aux_from(const pylbm_params* p) {
+  AuxArgs x{};
+  x.lattice = p->lattice;
+  x.storage = p->storage;
+  x.aa = p->aa;
+  x.N = (int)p->n_nodes;
+  x.ff = p->force_field;
+  x.has_force = (p->force_field != nullptr || p->force[0] != 0.0f || p->force[1] != 0.0f ||
+                 p->force[2] != 0.0f) ? 1 : 0;
+  for (int a = 0; a < 3; ++a) x.F[a] = p->force[a];
+  return x;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pylbm_hip_abi_version(void) { return PYLBM_HIP_ABI_VERSION; }
+
+const char* pylbm_hip_last_error(void) { return g_err.c_str(); }
+
+int pylbm_hip_step(const pylbm_params* p, const void* src, void* dst, int pattern,
+                   void* stream) {
+  if (int e = check_params(p, "pylbm_hip_step")) return e;
+  if (!src || !dst) return fail(PYLBM_EINVAL, "pylbm_hip_step: NULL buffer");
+  if (pattern == PYLBM_AB) {
+    if (src == dst) return fail(PYLBM_EINVAL, "pylbm_hip_step: AB needs src != dst");
+    if (p->aa) return fail(PYLBM_EINVAL, "pylbm_hip_step: AB pattern on an AA-layout state");
+  } else if (pattern == PYLBM_AA_ODD || pattern == PYLBM_AA_EVEN) {
+    if (src != dst) return fail(PYLBM_EINVAL, "pylbm_hip_step: AA patterns work in place");
+  } else {
+    return fail(PYLBM_EINVAL, "pylbm_hip_step: unknown pattern %d", pattern);
+  }
+  KArgs a;
+  if (int e = make_kargs(p, a, "pylbm_hip_step")) return e;
+  return hip_fail(step_impl(p, a, src, dst, pattern, (hipStream_t)stream), "pylbm_hip_step");
+}
+
+int pylbm_hip_run(const pylbm_params* p, void* a_buf, void* b_buf, int64_t nsteps,
+                  void* stream) {
+  if (int e = check_params(p, "pylbm_hip_run")) return e;
+  if (nsteps < 0) return fail(PYLBM_EINVAL, "pylbm_hip_run: nsteps < 0");
+  if (!a_buf) return fail(PYLBM_EINVAL, "pylbm_hip_run: NULL buffer");
+  KArgs a;
+  if (int e = make_kargs(p, a, "pylbm_hip_run")) return e;
+  hipStream_t s = (hipStream_t)stream;
+  if (p->aa) {
+    if (nsteps % 2) return fail(PYLBM_EINVAL, "pylbm_hip_run: AA advances in pairs of steps");
+    for (int64_t k = 0; k < nsteps; k += 2) {
+      if (int e = step_impl(p, a, a_buf, a_buf, PYLBM_AA_ODD, s))
+        return hip_fail(e, "pylbm_hip_run (aa-odd)");
+      if (int e = step_impl(p, a, a_buf, a_buf, PYLBM_AA_EVEN, s))
+        return hip_fail(e, "pylbm_hip_run (aa-even)");
+    }
+    return 0;
+  }
+  if (!b_buf || b_buf == a_buf)
+    return fail(PYLBM_EINVAL, "pylbm_hip_run: two-buffer stepping needs a distinct b");
+  void* src = a_buf;
+  void* dst = b_buf;
+  for (int64_t k = 0; k < nsteps; ++k) {
+    if (int e = step_impl(p, a, src, dst, PYLBM_AB, s)) return hip_fail(e, "pylbm_hip_run");
+    void* t = src;
+    src = dst;
+    dst = t;
+  }
+  return 0;
+}
+
+int pylbm_hip_encode(const pylbm_params* p, const double* canon, void* f, void* stream) {
+  if (int e = check_params(p, "pylbm_hip_encode")) return e;
+  if (!canon || !f) return fail(PYLBM_EINVAL, "pylbm_hip_encode: NULL buffer");
+  AuxArgs x = aux_from(p);
+  x.d_in = canon;
+  x.f_out = f;
+  return hip_fail(launch_aux(kOpEncode, x, (hipStream_t)stream), "pylbm_hip_encode");
+}
+
+int pylbm_hip_decode(const pylbm_params* p, const void* f, double* canon, void* stream) {
+  if (int e = check_params(p, "pylbm_hip_decode")) return e;
+  if (!canon || !f) return fail(PYLBM_EINVAL, "pylbm_hip_decode: NULL buffer");
+  AuxArgs x = aux_from(p);
+  x.f = f;
+  x.d_out = canon;
+  return hip_fail(launch_aux(kOpDecode, x, (hipStream_t)stream), "pylbm_hip_decode");
+}
+
+int pylbm_hip_init_equilibrium(const pylbm_params* p, const double* rho, const double* u,
+                               void* f, void* stream) {
+  if (int e = check_params(p, "pylbm_hip_init_equilibrium")) return e;
+  if (!f) return fail(PYLBM_EINVAL, "pylbm_hip_init_equilibrium: NULL buffer");
+  AuxArgs x = aux_from(p);
+  x.d_in = rho;
+  x.d_in2 = u;
+  x.f_out = f;
+  return hip_fail(launch_aux(kOpInit, x, (hipStream_t)stream), "pylbm_hip_init_equilibrium");
+}
+
+int pylbm_hip_macroscopic(const pylbm_params* p, const void* f, double* rho, double* u,
+                          void* stream) {
+  if (int e = check_params(p, "pylbm_hip_macroscopic")) return e;
+  if (!f) return fail(PYLBM_EINVAL, "pylbm_hip_macroscopic: NULL buffer");
+  if (!rho && !u) return 0;
+  AuxArgs x = aux_from(p);
+  x.f = f;
+  x.d_out = rho;
+  x.d_out2 = u;
+  return hip_fail(launch_aux(kOpMacro, x, (hipStream_t)stream), "pylbm_hip_macroscopic");
+}
+
+int pylbm_hip_reduce(const pylbm_params* p, int kind, const void* f, const void* other,
+                     double* out, void* stream) {
+  if (int e = check_params(p, "pylbm_hip_reduce")) return e;
+  if (!f || !out) return fail(PYLBM_EINVAL, "pylbm_hip_reduce: NULL buffer");
+  if (kind < PYLBM_RED_MASS || kind > PYLBM_RED_MAXABS_DIFF)
+    return fail(PYLBM_EINVAL, "pylbm_hip_reduce: unknown kind %d", kind);
+  if (kind == PYLBM_RED_MAXABS_DIFF && !other)
+    return fail(PYLBM_EINVAL, "pylbm_hip_reduce: MAXABS_DIFF needs a second state");
+  AuxArgs x = aux_from(p);
+  x.f = f;
+  x.f2 = other;
+  x.kind = kind;
+  x.d_out = out;
+  return hip_fail(launch_aux(kOpReduce, x, (hipStream_t)stream), "pylbm_hip_reduce");
+}
+
+int pylbm_hip_copy(const void* src, void* dst, int64_t nbytes, void* stream) {
+  if (!src || !dst) return fail(PYLBM_EINVAL, "pylbm_hip_copy: NULL buffer");
+  if (nbytes < 0 || nbytes % 16)
+    return fail(PYLBM_EINVAL, "pylbm_hip_copy: nbytes must be a non-negative multiple of 16");
+  if (nbytes == 0) return 0;
+  return hip_fail(launch_copy(src, dst, nbytes, (hipStream_t)stream), "pylbm_hip_copy");
+}
+
+}  // extern "C"

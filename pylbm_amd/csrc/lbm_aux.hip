@@ -1,0 +1,307 @@
+// Kernels on either side of the fused step: canonical-state encode / decode, device-side
+// equilibrium initialisation, float64 observables and reductions, and the streaming copy
+// that measures the bandwidth ceiling.
+//
+// The float64 arithmetic here is written with contraction off and in the reference's
+// evaluation order, so that results equal the reference's host computations on the same
+// populations bit for bit:
+//   encode  : pylbm/backends/_fused.py:83-90 (f - w_i in float64 -> float32 -> storage)
+//   decode  : pylbm/backends/_fused.py:95-103 (exact widening, + w_i, AA unswap)
+//   init    : pylbm/lattice.py:112-119, pylbm/problem.py:108-115
+//   moments : pylbm/core.py:151-169, 255-266
+#include "lbm_aux.hpp"
+
+#pragma clang fp contract(off)
+
+namespace plbm {
+
+namespace {
+
+template <class L, int ST>
+__global__ void __launch_bounds__(kBlock)
+k_decode(const typename Codec<ST>::T* __restrict__ A, double* __restrict__ out, int N, int aa) {
+  const int n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= N) return;
+#pragma unroll
+  for (int i = 0; i < L::Q; ++i) {
+    const int slot = aa ? L::opp(i) : i;
+    double v = (double)Codec<ST>::ld(A[(size_t)slot * N + n]);
+    if (Codec<ST>::shifted) v = v + L::w(i);
+    out[(size_t)i * N + n] = v;
+  }
+}
+
+template <class L, int ST>
+__device__ __forceinline__ void encode_node(typename Codec<ST>::T* A, const double (&f)[L::Q],
+                                            size_t N, int n, int aa) {
+#pragma unroll
+  for (int i = 0; i < L::Q; ++i) {
+    double v = f[i];
+    if (Codec<ST>::shifted) v = v - L::w(i);
+    const int slot = aa ? L::opp(i) : i;
+    A[(size_t)slot * N + n] = Codec<ST>::st((float)v);
+  }
+}
+
+template <class L, int ST>
+__global__ void __launch_bounds__(kBlock)
+k_encode(const double* __restrict__ in, typename Codec<ST>::T* __restrict__ A, int N, int aa) {
+  const int n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= N) return;
+  double f[L::Q];
+#pragma unroll
+  for (int i = 0; i < L::Q; ++i) f[i] = in[(size_t)i * N + n];
+  encode_node<L, ST>(A, f, (size_t)N, n, aa);
+}
+
+template <class L, int ST>
+__global__ void __launch_bounds__(kBlock)
+k_init_eq(const double* __restrict__ rho, const double* __restrict__ u,
+          typename Codec<ST>::T* __restrict__ A, int N, int aa) {
+  const int n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= N) return;
+  const double r = rho ? rho[n] : 1.0;
+  double uu[3] = {0.0, 0.0, 0.0};
+  if (u) {
+#pragma unroll
+    for (int a = 0; a < L::D; ++a) uu[a] = u[(size_t)a * N + n];
+  }
+  double usq = 0.0;
+#pragma unroll
+  for (int a = 0; a < L::D; ++a) usq = usq + uu[a] * uu[a];
+  double f[L::Q];
+#pragma unroll
+  for (int i = 0; i < L::Q; ++i) {
+    double cu = 0.0;
+#pragma unroll
+    for (int a = 0; a < L::D; ++a) cu = cu + (double)L::c(i, a) * uu[a];
+    f[i] = L::w(i) * r * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * usq);
+  }
+  encode_node<L, ST>(A, f, (size_t)N, n, aa);
+}
+
+// rho and u of the post-collision state at node n, in the reference's summation order.
+template <class L, int ST>
+__device__ __forceinline__ void node_moments(const typename Codec<ST>::T* A, size_t N, int n,
+                                             int aa, const double F[3], bool has_force,
+                                             const bool Fnz[3], double& rho, double (&u)[3]) {
+  double f[L::Q];
+#pragma unroll
+  for (int i = 0; i < L::Q; ++i) {
+    const int slot = aa ? L::opp(i) : i;
+    double v = (double)Codec<ST>::ld(A[(size_t)slot * N + n]);
+    if (Codec<ST>::shifted) v = v + L::w(i);
+    f[i] = v;
+  }
+  rho = f[0];
+#pragma unroll
+  for (int i = 1; i < L::Q; ++i) rho = rho + f[i];
+#pragma unroll
+  for (int a = 0; a < L::D; ++a) {
+    double j = 0.0;
+    bool first = true;
+#pragma unroll
+    for (int i = 0; i < L::Q; ++i) {
+      const int c = L::c(i, a);
+      if (c == 0) continue;
+      const double t = (c == 1) ? f[i] : -f[i];
+      j = first ? t : j + t;
+      first = false;
+    }
+    if (Fnz[a]) j = j + 0.5 * F[a];
+    u[a] = j / rho;
+  }
+  if (has_force) {
+#pragma unroll
+    for (int a = 0; a < L::D; ++a) u[a] = (u[a] * rho - F[a]) / rho;
+  }
+  if (L::D == 2) u[2] = 0.0;
+}
+
+struct MomArgs {
+  int N;
+  int aa;
+  int has_force;
+  double F[3];
+  int Fnz[3];
+  const float* ff;  // per-cell force field (d, N) or nullptr
+};
+
+template <class L, int ST>
+__device__ __forceinline__ void moments_at(const typename Codec<ST>::T* A, const MomArgs& m,
+                                           int n, double& rho, double (&u)[3]) {
+  double F[3] = {m.F[0], m.F[1], m.F[2]};
+  bool Fnz[3] = {m.Fnz[0] != 0, m.Fnz[1] != 0, m.Fnz[2] != 0};
+  if (m.ff) {
+#pragma unroll
+    for (int a = 0; a < L::D; ++a) {
+      F[a] = (double)m.ff[(size_t)a * m.N + n];
+      Fnz[a] = true;
+    }
+  }
+  node_moments<L, ST>(A, (size_t)m.N, n, m.aa, F, m.has_force != 0, Fnz, rho, u);
+}
+
+template <class L, int ST>
+__global__ void __launch_bounds__(kBlock)
+k_macro(const typename Codec<ST>::T* __restrict__ A, MomArgs m, double* __restrict__ rho_out,
+        double* __restrict__ u_out) {
+  const int n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= m.N) return;
+  double rho, u[3];
+  moments_at<L, ST>(A, m, n, rho, u);
+  if (rho_out) rho_out[n] = rho;
+  if (u_out) {
+#pragma unroll
+    for (int a = 0; a < L::D; ++a) u_out[(size_t)a * m.N + n] = u[a];
+  }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  return v;
+}
+
+template <bool MAX>
+__device__ void block_reduce_to(double v, double* out) {
+  __shared__ double part[kBlock / 64];
+  v = MAX ? wave_max(v) : wave_sum(v);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) part[wv] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = part[0];
+    for (int k = 1; k < kBlock / 64; ++k) t = MAX ? fmax(t, part[k]) : t + part[k];
+    if (MAX) {
+      // non-negative doubles order like their bit patterns
+      atomicMax((unsigned long long*)out, (unsigned long long)__double_as_longlong(t));
+    } else {
+      atomicAdd(out, t);
+    }
+  }
+}
+
+template <class L, int ST>
+__global__ void __launch_bounds__(kBlock)
+k_reduce(const typename Codec<ST>::T* __restrict__ A, const typename Codec<ST>::T* __restrict__ B,
+         MomArgs m, int kind, double* out) {
+  const int n = blockIdx.x * kBlock + threadIdx.x;
+  double v = 0.0;
+  if (n < m.N) {
+    if (kind == PYLBM_RED_MASS) {
+#pragma unroll
+      for (int i = 0; i < L::Q; ++i) {
+        double fv = (double)Codec<ST>::ld(A[(size_t)i * m.N + n]);
+        if (Codec<ST>::shifted) fv = fv + L::w(i);
+        v = v + fv;
+      }
+    } else if (kind == PYLBM_RED_KINETIC) {
+      double rho, u[3];
+      moments_at<L, ST>(A, m, n, rho, u);
+      v = 0.5 * rho * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    } else {
+#pragma unroll
+      for (int i = 0; i < L::Q; ++i) {
+        const double a = (double)Codec<ST>::ld(A[(size_t)i * m.N + n]);
+        const double b = (double)Codec<ST>::ld(B[(size_t)i * m.N + n]);
+        v = fmax(v, fabs(a - b));
+      }
+    }
+  }
+  if (kind == PYLBM_RED_MAXABS_DIFF) block_reduce_to<true>(v, out);
+  else block_reduce_to<false>(v, out);
+}
+
+__global__ void k_zero1(double* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *out = 0.0;
+}
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// Streaming copy: each lane moves 4 x 16 B, the four loads issued before the stores.
+__global__ void __launch_bounds__(kBlock)
+k_copy(const u32x4* __restrict__ src, u32x4* __restrict__ dst, int64_t n16) {
+  const int64_t base = (int64_t)blockIdx.x * (kBlock * 4) + threadIdx.x;
+  u32x4 v[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int64_t i = base + (int64_t)k * kBlock;
+    if (i < n16) v[k] = __builtin_nontemporal_load(&src[i]);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int64_t i = base + (int64_t)k * kBlock;
+    if (i < n16) __builtin_nontemporal_store(v[k], &dst[i]);
+  }
+}
+
+inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+template <class L, int ST>
+int aux_typed(int op, const AuxArgs& x, hipStream_t s) {
+  using T = typename Codec<ST>::T;
+  const int N = x.N;
+  const dim3 g(nblocks(N)), b(kBlock);
+  MomArgs m{};
+  m.N = N;
+  m.aa = x.aa;
+  m.has_force = x.has_force;
+  for (int a = 0; a < 3; ++a) {
+    m.F[a] = (double)x.F[a];
+    m.Fnz[a] = x.F[a] != 0.0f;
+  }
+  m.ff = x.ff;
+  switch (op) {
+    case kOpDecode:
+      k_decode<L, ST><<<g, b, 0, s>>>((const T*)x.f, x.d_out, N, x.aa);
+      break;
+    case kOpEncode:
+      k_encode<L, ST><<<g, b, 0, s>>>(x.d_in, (T*)x.f_out, N, x.aa);
+      break;
+    case kOpInit:
+      k_init_eq<L, ST><<<g, b, 0, s>>>(x.d_in, x.d_in2, (T*)x.f_out, N, x.aa);
+      break;
+    case kOpMacro:
+      k_macro<L, ST><<<g, b, 0, s>>>((const T*)x.f, m, x.d_out, x.d_out2);
+      break;
+    case kOpReduce:
+      k_zero1<<<1, 64, 0, s>>>(x.d_out);
+      k_reduce<L, ST><<<g, b, 0, s>>>((const T*)x.f, (const T*)x.f2, m, x.kind, x.d_out);
+      break;
+    default:
+      return PYLBM_EINVAL;
+  }
+  return (int)hipGetLastError();
+}
+
+template <class L>
+int aux_lat(int op, const AuxArgs& x, hipStream_t s) {
+  switch (x.storage) {
+    case PYLBM_F32: return aux_typed<L, PYLBM_F32>(op, x, s);
+    case PYLBM_F16: return aux_typed<L, PYLBM_F16>(op, x, s);
+    case PYLBM_BF16: return aux_typed<L, PYLBM_BF16>(op, x, s);
+  }
+  return PYLBM_EINVAL;
+}
+
+}  // namespace
+
+int launch_aux(int op, const AuxArgs& x, hipStream_t s) {
+  return x.lattice == PYLBM_D2Q9 ? aux_lat<D2Q9>(op, x, s) : aux_lat<D3Q19>(op, x, s);
+}
+
+int launch_copy(const void* src, void* dst, int64_t nbytes, hipStream_t s) {
+  const int64_t n16 = nbytes / 16;
+  const int64_t per_block = (int64_t)kBlock * 4;
+  const dim3 g((unsigned)((n16 + per_block - 1) / per_block)), b(kBlock);
+  k_copy<<<g, b, 0, s>>>((const u32x4*)src, (u32x4*)dst, n16);
+  return (int)hipGetLastError();
+}
+
+}  // namespace plbm

@@ -1,0 +1,29 @@
+// Host-side interface of the auxiliary kernels (encode / decode / init / observables / copy).
+#pragma once
+
+#include "lbm_common.hpp"
+
+namespace plbm {
+
+enum AuxOp { kOpDecode = 0, kOpEncode = 1, kOpInit = 2, kOpMacro = 3, kOpReduce = 4 };
+
+struct AuxArgs {
+  int lattice, storage, aa;
+  int N;
+  int has_force;
+  float F[3];
+  const float* ff;
+  int kind;                 // reduction kind
+  const void* f;            // stored state (input)
+  const void* f2;           // second stored state (MAXABS_DIFF)
+  void* f_out;              // stored state (output)
+  const double* d_in;       // canonical state / rho
+  const double* d_in2;      // u
+  double* d_out;            // canonical state / rho / reduction result
+  double* d_out2;           // u
+};
+
+int launch_aux(int op, const AuxArgs& x, hipStream_t s);
+int launch_copy(const void* src, void* dst, int64_t nbytes, hipStream_t s);
+
+}  // namespace plbm

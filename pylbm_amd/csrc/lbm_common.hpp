@@ -1,0 +1,192 @@
+// Shared device-side definitions of the gfx950 fused LBM kernels: lattice traits, storage
+// codecs, the per-node collision (BGK / TRT / MRT with Guo forcing) and kernel arguments.
+//
+// Lattice orderings, weights and the D2Q9 MRT basis are the reference's
+// (pylbm/lattice.py:49-87); the collision arithmetic restates pylbm/core.py:151-247 in
+// fp32, the precision of the reference's fused kernels (pylbm/backends/cuda/lbm.cu:100-163).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/pylbm_hip.h"
+
+namespace plbm {
+
+constexpr int kBlock = 256;  // 4 wave64 per workgroup
+
+// ------------------------------------------------------------------------ lattices
+struct D2Q9 {
+  static constexpr int Q = 9, D = 2, ID = PYLBM_D2Q9;
+  __host__ __device__ static constexpr int c(int i, int a) {
+    constexpr int t[9][3] = {{0, 0, 0},  {1, 0, 0},  {0, 1, 0},   {-1, 0, 0}, {0, -1, 0},
+                             {1, 1, 0},  {-1, 1, 0}, {-1, -1, 0}, {1, -1, 0}};
+    return t[i][a];
+  }
+  __host__ __device__ static constexpr int opp(int i) {
+    constexpr int t[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+    return t[i];
+  }
+  __host__ __device__ static constexpr double w(int i) {
+    return i == 0 ? 4.0 / 9.0 : (i < 5 ? 1.0 / 9.0 : 1.0 / 36.0);
+  }
+};
+
+struct D3Q19 {
+  static constexpr int Q = 19, D = 3, ID = PYLBM_D3Q19;
+  __host__ __device__ static constexpr int c(int i, int a) {
+    constexpr int t[19][3] = {{0, 0, 0},   {1, 0, 0},  {-1, 0, 0}, {0, 1, 0},  {0, -1, 0},
+                              {0, 0, 1},   {0, 0, -1}, {1, 1, 0},  {-1, -1, 0}, {1, -1, 0},
+                              {-1, 1, 0},  {1, 0, 1},  {-1, 0, -1}, {1, 0, -1}, {-1, 0, 1},
+                              {0, 1, 1},   {0, -1, -1}, {0, 1, -1}, {0, -1, 1}};
+    return t[i][a];
+  }
+  __host__ __device__ static constexpr int opp(int i) {
+    return i == 0 ? 0 : ((i & 1) ? i + 1 : i - 1);
+  }
+  __host__ __device__ static constexpr double w(int i) {
+    return i == 0 ? 1.0 / 3.0 : (i < 7 ? 1.0 / 18.0 : 1.0 / 36.0);
+  }
+};
+
+// D2Q9 Lallemand-Luo basis, rows (rho, e, eps, jx, qx, jy, qy, pxx, pxy).
+__host__ __device__ constexpr int mrt_m(int k, int i) {
+  constexpr int t[9][9] = {{1, 1, 1, 1, 1, 1, 1, 1, 1},      {-4, -1, -1, -1, -1, 2, 2, 2, 2},
+                           {4, -2, -2, -2, -2, 1, 1, 1, 1},  {0, 1, 0, -1, 0, 1, -1, -1, 1},
+                           {0, -2, 0, 2, 0, 1, -1, -1, 1},   {0, 0, 1, 0, -1, 1, 1, -1, -1},
+                           {0, 0, -2, 0, 2, 1, 1, -1, -1},   {0, 1, -1, 1, -1, 0, 0, 0, 0},
+                           {0, 0, 0, 0, 0, 1, -1, 1, -1}};
+  return t[k][i];
+}
+__host__ __device__ constexpr int mrt_norm(int k) {
+  constexpr int t[9] = {9, 36, 36, 6, 12, 6, 12, 4, 4};
+  return t[k];
+}
+
+// ------------------------------------------------------------------------ storage codecs
+template <int ST> struct Codec;
+
+template <> struct Codec<PYLBM_F32> {
+  using T = float;
+  static constexpr bool shifted = false;
+  __device__ static __forceinline__ float ld(T v) { return v; }
+  __device__ static __forceinline__ T st(float v) { return v; }
+};
+
+template <> struct Codec<PYLBM_F16> {
+  using T = _Float16;
+  static constexpr bool shifted = true;
+  __device__ static __forceinline__ float ld(T v) { return (float)v; }
+  // v_cvt_f16_f32 rounds to nearest even in the default mode.
+  __device__ static __forceinline__ T st(float v) { return (_Float16)v; }
+};
+
+template <> struct Codec<PYLBM_BF16> {
+  using T = uint16_t;
+  static constexpr bool shifted = true;
+  __device__ static __forceinline__ float ld(T v) {
+    return __uint_as_float(((uint32_t)v) << 16);
+  }
+  // float -> bfloat16, round to nearest even (gfx950 converts in hardware).
+  __device__ static __forceinline__ T st(float v) {
+    __bf16 b = (__bf16)v;
+    return __builtin_bit_cast(uint16_t, b);
+  }
+};
+
+// ------------------------------------------------------------------------ kernel args
+struct KArgs {
+  int nx, ny, nz;
+  int N;       // nodes per plane (< 2^31)
+  int nseg;    // workgroups per row along x
+  int nblk;    // total workgroups
+  int per;     // bit a: axis a periodic
+  int remap;   // 1: XCD-aware workgroup -> row-segment mapping
+  float ulid;
+  float fx, fy, fz;
+  float r0, r1, r2, r3;
+  const float* ff;  // per-cell force (d, N) or nullptr
+};
+
+// Workgroups are dispatched round-robin over the 8 XCDs.  With remap, XCD k processes the
+// k-th contiguous eighth of the row segments, so neighbouring segments (which share the
+// cache lines at their x edges) and neighbouring rows share one L2.
+__device__ __forceinline__ int remap_block(int b, int nblk, int remap) {
+  if (!remap) return b;
+  const int per_xcd = nblk >> 3;  // nblk % 8 == 0 when remap is set
+  return (b & 7) * per_xcd + (b >> 3);
+}
+
+// ------------------------------------------------------------------------ collision
+// Per-node collision in fp32.  f: post-streaming populations; F: Guo force at the node
+// (zero when the problem has no force, which leaves every term exact); returns p.
+template <class L, int COLL>
+__device__ __forceinline__ void collide_node(const float (&f)[L::Q], float (&p)[L::Q],
+                                             const float F[3], bool force, float r0, float r1,
+                                             float r2, float r3) {
+  constexpr int Q = L::Q;
+  float rho = 0.f, j0 = 0.f, j1 = 0.f, j2 = 0.f;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    rho += f[i];
+    if (L::c(i, 0)) j0 += L::c(i, 0) * f[i];
+    if (L::c(i, 1)) j1 += L::c(i, 1) * f[i];
+    if (L::D == 3 && L::c(i, 2)) j2 += L::c(i, 2) * f[i];
+  }
+  const float inv = 1.0f / rho;
+  const float ux = (j0 + 0.5f * F[0]) * inv;
+  const float uy = (j1 + 0.5f * F[1]) * inv;
+  const float uz = (L::D == 3) ? (j2 + 0.5f * F[2]) * inv : 0.0f;
+  const float usq = ux * ux + uy * uy + uz * uz;
+  const float base = 1.0f - 1.5f * usq;
+  const float uf = ux * F[0] + uy * F[1] + uz * F[2];
+  float feq[Q], S[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const float cu = L::c(i, 0) * ux + L::c(i, 1) * uy + L::c(i, 2) * uz;
+    const float w = (float)L::w(i);
+    feq[i] = w * rho * (base + cu * (3.0f + 4.5f * cu));
+    const float cf = L::c(i, 0) * F[0] + L::c(i, 1) * F[1] + L::c(i, 2) * F[2];
+    S[i] = force ? w * (3.0f * (cf - uf) + 9.0f * cu * cf) : 0.0f;
+  }
+  if constexpr (COLL == PYLBM_BGK) {
+    const float g = 1.0f - 0.5f * r0;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) p[i] = f[i] + r0 * (feq[i] - f[i]) + g * S[i];
+  } else if constexpr (COLL == PYLBM_TRT) {
+    const float gp = 1.0f - 0.5f * r0, gm = 1.0f - 0.5f * r1;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const int o = L::opp(i);
+      const float np_ = 0.5f * (f[i] + f[o]) - 0.5f * (feq[i] + feq[o]);
+      const float nm_ = 0.5f * (f[i] - f[o]) - 0.5f * (feq[i] - feq[o]);
+      const float sp_ = 0.5f * (S[i] + S[o]), sm_ = 0.5f * (S[i] - S[o]);
+      p[i] = f[i] - r0 * np_ - r1 * nm_ + gp * sp_ + gm * sm_;
+    }
+  } else {
+    static_assert(COLL != PYLBM_MRT || L::Q == 9, "MRT is D2Q9 only");
+    const float s[9] = {0.f, r1, r2, 0.f, r3, 0.f, r3, r0, r0};
+    float dm[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      float mne = 0.f, ms = 0.f;
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        if (mrt_m(k, i) == 0) continue;
+        mne += (float)mrt_m(k, i) * (f[i] - feq[i]);
+        ms += (float)mrt_m(k, i) * S[i];
+      }
+      dm[k] = (-s[k] * mne + (1.0f - 0.5f * s[k]) * ms) / (float)mrt_norm(k);
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      float acc = f[i];
+#pragma unroll
+      for (int k = 0; k < 9; ++k)
+        if (mrt_m(k, i) != 0) acc += (float)mrt_m(k, i) * dm[k];
+      p[i] = acc;
+    }
+  }
+}
+
+}  // namespace plbm

@@ -1,0 +1,142 @@
+"""HIP kernels vs the float64 oracle (GPU).
+
+Mirrors the reference's backend-equivalence tests (tests/test_equivalence.py:34-62): every
+``hip`` variant reproduces the reference trajectories of the seven box cases after 100
+steps (golden vectors written from the reference NumPy solver), within the reference's
+tolerances; the AA pattern equals the two-buffer pattern; plus the config-level checks:
+the 128^2 cavity after 1000 steps and the periodic 256^2 shear wave (BGK/TRT/MRT, pull and
+AA) within 1e-5 of float64.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import lbm_ref as R
+from pylbm_amd import backends as B
+from pylbm_amd.problem import Problem
+from tests.cases import EQUIVALENCE, TOL, golden, shear_wave_fields
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = [v for _, v in B.entries()]
+
+
+def _make(variant: str, p: Problem):
+    cls = B.get("hip", variant)
+    ok, why = cls.available()
+    assert ok, f"hip/{variant} must be available on a GPU box: {why}"
+    return cls(p)
+
+
+@pytest.fixture(scope="module")
+def eq_golden():
+    return golden("equivalence.npz")
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("case", list(EQUIVALENCE))
+def test_matches_reference_trajectory(variant, case, eq_golden):
+    p = Problem(**EQUIVALENCE[case])
+    s = _make(variant, p)
+    s.step(100)
+    f = s.populations()
+    err = np.abs(f - eq_golden[case]).max()
+    print(f"{case} hip/{variant}: max|df| = {err:.3e}")
+    assert err < TOL[s.storage]
+
+
+@pytest.mark.parametrize("variant", ["fused", "fused-aa"])
+def test_cavity128_1000_steps(variant):
+    """Config 1: D2Q9 BGK cavity 128^2, tau = 0.884, U = 0.1, 1000 steps, max|df| <= 1e-5."""
+    g = golden("cavity128.npz")
+    p = Problem.cavity(128, re=100.0, u_lid=0.1)
+    s = _make(variant, p)
+    s.step(1000)
+    f = s.populations()
+    err = max(np.abs(f[:, ::4, ::4] - g["sub"]).max(), np.abs(f[:, -1, :] - g["lid_row"]).max(),
+              np.abs(f[:, :, 0] - g["left_col"]).max())
+    print(f"cavity128 hip/{variant}: max|df| = {err:.3e}")
+    assert err <= 1e-5
+    assert np.allclose(f.sum(axis=(1, 2)), g["sum"], rtol=1e-6)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+def test_aa_pattern_equals_two_buffer(storage):
+    """tests/test_equivalence.py:55-62: AA and two-buffer agree (< 1e-6 in fp32)."""
+    suffix = "" if storage == "f32" else "-" + {"f16": "fp16", "bf16": "bf16"}[storage]
+    p = Problem((64, 40), tau=0.6, u_lid=0.1, collision="trt")
+    a, b = _make("fused" + suffix, p), _make("fused-aa" + suffix, p)
+    a.step(60)
+    b.step(60)
+    err = np.abs(a.populations() - b.populations()).max()
+    print(f"AA vs AB {storage}: {err:.3e}")
+    assert err < {"f32": 1e-6, "f16": 2e-4, "bf16": 2e-3}[storage]
+
+
+@pytest.mark.parametrize("coll,extra", [("bgk", {}), ("trt", {}),
+                                        ("mrt", {"s_e": 1.2, "s_eps": 1.2}),
+                                        ("mrt", {"s_e": 1.2, "s_eps": 1.2, "s_q": 1.2})],
+                         ids=["bgk", "trt", "mrt", "mrt-sq1.2"])
+@pytest.mark.parametrize("variant", ["fused", "fused-aa"])
+def test_periodic_shear_wave_256(variant, coll, extra):
+    """Config 2 same-run check: periodic 256^2 shear wave, tau(+) = 0.6, 200 steps."""
+    n = 256
+    rho, u = shear_wave_fields(n, seed=7)
+    kw = dict(shape=(n, n), tau=0.6, collision=coll, periodic=(True, True), init_rho=rho,
+              init_u=u, **extra)
+    ref = R.run(R.Case(**kw), 200)
+    s = _make(variant, Problem(**kw))
+    s.step(200)
+    err = np.abs(s.populations() - ref).max()
+    print(f"shear256 {coll}{extra} hip/{variant}: max|df| = {err:.3e}")
+    assert err <= 1e-5
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_state_roundtrip_and_observables(variant):
+    """encode/decode, device init and float64 observables equal the host computations."""
+    rng = np.random.default_rng(3)
+    # forces exact in fp32, so the float64 host formulas see the value the kernels see
+    kw = dict(shape=(37, 19), tau=0.8, periodic=(True, False), force=(2.0**-15, -(2.0**-16)))
+    p = Problem(**kw)
+    case = R.Case(**kw)
+    lat = case.lat
+    f = lat.w[:, None, None] * (1.0 + 0.05 * rng.standard_normal((9, 19, 37)))
+    s = _make(variant, p)
+    s.set_populations(f)
+    got = s.populations()
+    want = R.decode(R.encode(f, s.storage, s.aa, lat), s.storage, s.aa, lat)
+    assert np.array_equal(got, want)
+    rho, u = s.macroscopic()
+    rho_o, u_o = R.macroscopic(got, case)
+    assert np.array_equal(rho, rho_o) and np.array_equal(u, u_o)
+    assert abs(s.reduce("mass") - got.sum()) < 1e-9
+    # device equilibrium init == host equilibrium + encode
+    rho0 = 1.0 + 0.01 * rng.standard_normal((19, 37))
+    u0 = 0.02 * rng.standard_normal((2, 19, 37))
+    s.init_equilibrium(rho0, u0)
+    want = R.decode(R.encode(R.equilibrium(lat, rho0, u0), s.storage, s.aa, lat),
+                    s.storage, s.aa, lat)
+    assert np.array_equal(s.populations(), want)
+
+
+def test_per_cell_force_field_matches_oracle():
+    """Per-cell Guo force (extension): HIP vs oracle on a channel, 100 steps."""
+    rng = np.random.default_rng(11)
+    nx, ny = 24, 16
+    ff = np.zeros((2, ny, nx))
+    ff[0] = 1e-5 + 1e-6 * rng.standard_normal((ny, nx))
+    ff[1] = 1e-6 * rng.standard_normal((ny, nx))
+    ff = ff.astype(np.float32).astype(np.float64)  # the kernels read the field in fp32
+    for coll in ("bgk", "trt"):
+        kw = dict(shape=(nx, ny), tau=0.8, periodic=(True, False), collision=coll,
+                  force_field=ff)
+        ref = R.run(R.Case(**kw), 100)
+        for variant in ("fused", "fused-aa"):
+            s = _make(variant, Problem(**kw))
+            s.step(100)
+            err = np.abs(s.populations() - ref).max()
+            print(f"force field {coll} hip/{variant}: {err:.3e}")
+            assert err < 3e-6

@@ -8,6 +8,8 @@
  *   - D2Q9 order: 0 rest, E, N, W, S, NE, NW, SW, SE; D3Q19 opposite pairs adjacent
  *     (pylbm/lattice.py:49-67);
  *   - storage PYLBM_F32 keeps f; PYLBM_F16 / PYLBM_BF16 keep h = f - w_i (arithmetic fp32);
+ *     PYLBM_F32S keeps h = f - w_i in fp32 (no reference counterpart: the precision the
+ *     differentiable rollouts use for the non-equilibrium part);
  *   - AA layout (params.aa = 1): the canonical post-collision population i sits in slot
  *     opp(i) (pylbm/backends/_fused.py:83-103);
  *   - the moving lid is the top wall of the last axis, moving in +x (Ladd term).
@@ -30,7 +32,7 @@ extern "C" {
 
 enum { PYLBM_D2Q9 = 0, PYLBM_D3Q19 = 1 };
 enum { PYLBM_BGK = 0, PYLBM_TRT = 1, PYLBM_MRT = 2 };
-enum { PYLBM_F32 = 0, PYLBM_F16 = 1, PYLBM_BF16 = 2 };
+enum { PYLBM_F32 = 0, PYLBM_F16 = 1, PYLBM_BF16 = 2, PYLBM_F32S = 3 };
 enum { PYLBM_AB = 0, PYLBM_AA_ODD = 1, PYLBM_AA_EVEN = 2 };
 enum { PYLBM_RED_MASS = 0, PYLBM_RED_KINETIC = 1, PYLBM_RED_MAXABS_DIFF = 2 };
 
@@ -50,7 +52,7 @@ enum {
 typedef struct pylbm_params {
   int32_t lattice;      /* PYLBM_D2Q9 | PYLBM_D3Q19 */
   int32_t collision;    /* PYLBM_BGK | PYLBM_TRT | PYLBM_MRT (D2Q9 only) */
-  int32_t storage;      /* PYLBM_F32 | PYLBM_F16 | PYLBM_BF16 */
+  int32_t storage;      /* PYLBM_F32 | PYLBM_F16 | PYLBM_BF16 | PYLBM_F32S */
   int32_t aa;           /* 1: AA layout (slot opp(i) holds population i), else 0 */
   int32_t periodic[3];  /* per axis x, y, z: 1 periodic, 0 half-way bounce-back walls */
   int32_t nx, ny, nz;   /* nz = 1 for D2Q9 */
@@ -134,6 +136,38 @@ int pylbm_hip_reduce(const pylbm_params* p, int kind, const void* f, const void*
  * Replaces the torch copy_ bandwidth probe (bench/roofline.py:30-51).
  */
 int pylbm_hip_copy(const void* src, void* dst, int64_t nbytes, void* stream);
+
+/* ---------------------------------------------------------------------------------------
+ * Discrete adjoint of the two-buffer step (BGK / TRT, Guo forcing; params->aa must be 0).
+ * No reference counterpart: the reference has no gradient path (its torch rungs run under
+ * torch.no_grad(), pylbm/backends/torch_backend.py:74,128,152,175,257).  Used by the
+ * checkpointed rollout behind torch.autograd.Function (pylbm_amd/adjoint.py):
+ *   seed -> for t = T-1 .. 0: adjoint_step(f^t) -> init.
+ * Adjoint states lam are fp32 (q, N); dF is fp32 (d, N), accumulated in place.
+ */
+
+/* Doubles of the per-workgroup dL/ds+- partial-sum buffer pylbm_hip_adjoint_step needs
+ * (2 per workgroup), or -1 on bad params. */
+int64_t pylbm_hip_adjoint_workspace(const pylbm_params* p);
+
+/* Terminal adjoint of a loss L(u(f_T)) given grad_u = dL/du (fp32, (d, N)), with
+ * u = (sum c f - F/2)/rho the post-collision velocity: writes lam = dL/df_T and, with a
+ * force field, initialises dF = dL/dF of that velocity definition (-grad_u / (2 rho)). */
+int pylbm_hip_adjoint_seed(const pylbm_params* p, const void* f_T, const float* grad_u,
+                           float* lam, float* dF, void* stream);
+
+/* One adjoint step: lam_out = (d f^{t+1} / d f^t)^T lam_in, with f_t the stored state the
+ * forward step read.  Accumulates dF += dL/dF(x) (force field only) and adds the step's
+ * dL/ds+ and dL/ds- (BGK: both into slot 0 pattern [s+, 0]) to dtau_part[2 g], [2 g + 1]
+ * for workgroup g; the caller zeroes dtau_part once and sums it at the end. */
+int pylbm_hip_adjoint_step(const pylbm_params* p, const void* f_t, const float* lam_in,
+                           float* lam_out, float* dF, double* dtau_part, void* stream);
+
+/* Gradient of the initial equilibrium f0 = feq(rho0, u0) (float64 fields; NULL rho0 = 1,
+ * NULL u0 = 0): du0 (d, N) and drho0 (N) in float64 from lam0 = dL/df0 (either output may
+ * be NULL). */
+int pylbm_hip_adjoint_init(const pylbm_params* p, const double* rho0, const double* u0,
+                           const float* lam0, double* du0, double* drho0, void* stream);
 
 #ifdef __cplusplus
 }

@@ -384,7 +384,7 @@ def encode(f: np.ndarray, storage: str, aa: bool, lat: Lat) -> np.ndarray:
     if aa:
         g = g[lat.opp]
     g32 = g.astype(np.float32)
-    if storage == "f32":
+    if storage in ("f32", "f32s"):  # f32s: shifted fp32 (no reference counterpart)
         return g32
     if storage == "f16":
         return g32.astype(np.float16)

@@ -16,13 +16,13 @@ ABI_VERSION = 1
 
 D2Q9, D3Q19 = 0, 1
 BGK, TRT, MRT = 0, 1, 2
-F32, F16, BF16 = 0, 1, 2
+F32, F16, BF16, F32S = 0, 1, 2, 3
 AB, AA_ODD, AA_EVEN = 0, 1, 2
 RED_MASS, RED_KINETIC, RED_MAXABS_DIFF = 0, 1, 2
 
 LATTICE_IDS = {"D2Q9": D2Q9, "D3Q19": D3Q19}
 COLLISION_IDS = {"bgk": BGK, "trt": TRT, "mrt": MRT}
-STORAGE_IDS = {"f32": F32, "f16": F16, "bf16": BF16}
+STORAGE_IDS = {"f32": F32, "f16": F16, "bf16": BF16, "f32s": F32S}
 PATTERN_IDS = {"ab": AB, "aa-odd": AA_ODD, "aa-even": AA_EVEN}
 
 #: every symbol include/pylbm_hip.h declares
@@ -30,6 +30,8 @@ EXPORTS = (
     "pylbm_hip_abi_version", "pylbm_hip_last_error", "pylbm_hip_step", "pylbm_hip_run",
     "pylbm_hip_encode", "pylbm_hip_decode", "pylbm_hip_init_equilibrium",
     "pylbm_hip_macroscopic", "pylbm_hip_reduce", "pylbm_hip_copy",
+    "pylbm_hip_adjoint_workspace", "pylbm_hip_adjoint_seed", "pylbm_hip_adjoint_step",
+    "pylbm_hip_adjoint_init",
 )
 
 
@@ -87,6 +89,10 @@ def load() -> Any:
         "pylbm_hip_macroscopic": ([pp, vp, vp, vp, vp], i32),
         "pylbm_hip_reduce": ([pp, i32, vp, vp, vp, vp], i32),
         "pylbm_hip_copy": ([vp, vp, i64, vp], i32),
+        "pylbm_hip_adjoint_workspace": ([pp], i64),
+        "pylbm_hip_adjoint_seed": ([pp, vp, vp, vp, vp, vp], i32),
+        "pylbm_hip_adjoint_step": ([pp, vp, vp, vp, vp, vp, vp], i32),
+        "pylbm_hip_adjoint_init": ([pp, vp, vp, vp, vp, vp, vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

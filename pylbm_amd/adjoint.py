@@ -1,0 +1,236 @@
+"""Differentiable K-step rollouts on the GPU: checkpointed forward + fused HIP adjoint.
+
+:class:`Rollout` advances a box problem ``steps`` steps with the two-buffer fused kernels
+(fp32, fp16 or bf16 storage) and returns the final post-collision velocity
+u = (sum c f - F/2) / rho.  :func:`rollout` wraps it in ``torch.autograd.Function`` so any
+torch loss of u back-propagates to
+
+* the per-cell Guo force field F (d, *array_shape),
+* tau (BGK omega = 1/tau; TRT s+ = 1/tau, s- from the magic parameter), and
+* the initial velocity (and density) of the equilibrium start state.
+
+Forward states are checkpointed in HBM every ``checkpoint`` steps; the backward pass
+recomputes one segment at a time (all its states kept in HBM), then runs the adjoint steps
+of that segment in reverse with ``pylbm_hip_adjoint_step``.  The last segment's states are
+kept from the forward pass, so it is not recomputed.  The reference has no gradient path;
+the gradients are checked against float64 torch autograd of a restated step and against
+central finite differences.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from typing import Any
+
+import torch
+
+from . import _lib
+from .problem import Problem
+
+_TORCH_DTYPES = {"f32": torch.float32, "f32s": torch.float32, "f16": torch.float16,
+                 "bf16": torch.bfloat16}
+
+
+def _p(t: Any) -> int | None:
+    return None if t is None else int(t.data_ptr())
+
+
+def _s() -> int:
+    return int(torch.cuda.current_stream().cuda_stream)
+
+
+def tau_rate_derivatives(problem: Problem, tau: float) -> tuple[float, float]:
+    """(ds+/dtau, ds-/dtau) of the rates the kernels use (BGK: (d omega/dtau, 0))."""
+    dsp = -1.0 / tau ** 2
+    if problem.collision == "bgk":
+        return dsp, 0.0
+    lam = problem.magic
+    tau_m = 0.5 + lam / (tau - 0.5)
+    return dsp, lam / ((tau - 0.5) ** 2 * tau_m ** 2)
+
+
+@dataclass
+class RolloutConfig:
+    """Geometry, collision and storage of a differentiable rollout."""
+
+    problem: Problem  # box, collision bgk|trt; its force_field marks a force-field rollout
+    steps: int
+    checkpoint: int = 100
+    #: "f32s" (default) stores f - w in fp32: the same 4 bytes as "f32", but the
+    #: non-equilibrium part that tau acts on keeps its digits; "f16"/"bf16" halve the bytes
+    storage: str = "f32s"
+
+    def __post_init__(self) -> None:
+        p = self.problem
+        if p.collision not in ("bgk", "trt"):
+            raise ValueError("the adjoint covers BGK and TRT")
+        if not p.is_box():
+            raise ValueError("box problems only")
+        if self.steps < 1 or self.checkpoint < 1:
+            raise ValueError("steps and checkpoint must be >= 1")
+
+
+class _Engine:
+    """Device buffers and C-ABI calls of one rollout configuration."""
+
+    def __init__(self, cfg: RolloutConfig, force_field: torch.Tensor | None,
+                 tau: float) -> None:
+        self.cfg = cfg
+        p = cfg.problem
+        self.p = p
+        self.q, self.d, self.N = p.lattice.q, p.dim, p.n_nodes
+        self.dev = torch.device("cuda", torch.cuda.current_device())
+        self.dtype = _TORCH_DTYPES[cfg.storage]
+        self.tau = float(tau)
+        self.ff = force_field
+        prm = _lib.Params()
+        prm.lattice = _lib.LATTICE_IDS[p.lattice.name]
+        prm.collision = _lib.COLLISION_IDS[p.collision]
+        prm.storage = _lib.STORAGE_IDS[cfg.storage]
+        prm.aa = 0
+        per = tuple(p.periodic) + (True,) * (3 - p.dim)
+        shp = tuple(p.shape) + (1,) * (3 - p.dim)
+        for k in range(3):
+            prm.periodic[k] = int(per[k])
+        prm.nx, prm.ny, prm.nz = shp
+        prm.n_nodes = self.N
+        prm.ulid = float(p.u_lid)
+        force = tuple(p.force) + (0.0,) * (3 - p.dim)
+        for k in range(3):
+            prm.force[k] = force[k]
+        rates = Problem(p.shape, tau=self.tau, collision=p.collision, magic=p.magic,
+                        periodic=p.periodic).runtime_rates()
+        for k, r in enumerate(rates):
+            prm.rates[k] = r
+        prm.force_field = _p(force_field)
+        self.prm = prm
+        self.ref = ctypes.byref(prm)
+
+    def state(self) -> torch.Tensor:
+        return torch.empty(self.q * self.N, device=self.dev, dtype=self.dtype)
+
+    def init(self, rho0: torch.Tensor | None, u0: torch.Tensor | None, f: torch.Tensor) -> None:
+        _lib.call("pylbm_hip_init_equilibrium", self.ref, _p(rho0), _p(u0), _p(f), _s())
+
+    def step(self, src: torch.Tensor, dst: torch.Tensor) -> None:
+        _lib.call("pylbm_hip_step", self.ref, _p(src), _p(dst), _lib.AB, _s())
+
+    def velocity(self, f: torch.Tensor) -> torch.Tensor:
+        u = torch.empty((self.d,) + self.p.array_shape, dtype=torch.float64, device=self.dev)
+        _lib.call("pylbm_hip_macroscopic", self.ref, _p(f), None, _p(u), _s())
+        return u
+
+
+class _RolloutFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, force_field, tau, u0, rho0, cfg):  # noqa: ANN001
+        ff = None if force_field is None else force_field.detach().to(
+            torch.float32).contiguous()
+        eng = _Engine(cfg, ff, float(tau))
+        u0d = None if u0 is None else u0.detach().to(torch.float64).contiguous()
+        r0d = None if rho0 is None else rho0.detach().to(torch.float64).contiguous()
+        T, C = cfg.steps, cfg.checkpoint
+        K = (T + C - 1) // C
+        last0 = (K - 1) * C  # first step of the last segment
+        f0 = eng.state()
+        eng.init(r0d, u0d, f0)
+        ckpts = [f0]          # f^{kC}, k = 0 .. K-1
+        last = []             # f^t, t = last0 .. T-1 (inputs of the last segment's steps)
+        scratch = [eng.state(), eng.state()]
+        cur = f0
+        for t in range(T):
+            if t >= last0:
+                last.append(cur)
+            keep = (t + 1 == T) or ((t + 1) % C == 0) or (t + 1 >= last0)
+            nxt = eng.state() if keep else (scratch[0] if cur is not scratch[0] else scratch[1])
+            eng.step(cur, nxt)
+            if t + 1 < T and (t + 1) % C == 0:
+                ckpts.append(nxt)
+            cur = nxt
+        del scratch
+        fT = cur
+        u = eng.velocity(fT)
+        ctx.eng, ctx.ckpts, ctx.last, ctx.fT = eng, ckpts, last, fT
+        ctx.u0d, ctx.r0d = u0d, r0d
+        ctx.needs = (force_field is not None and ctx.needs_input_grad[0],
+                     ctx.needs_input_grad[1], u0 is not None and ctx.needs_input_grad[2],
+                     rho0 is not None and ctx.needs_input_grad[3])
+        return u
+
+    @staticmethod
+    def backward(ctx, grad_u):  # noqa: ANN001
+        eng: _Engine = ctx.eng
+        cfg = eng.cfg
+        T, C = cfg.steps, cfg.checkpoint
+        K = (T + C - 1) // C
+        q, d, N = eng.q, eng.d, eng.N
+        G = grad_u.detach().to(torch.float32).contiguous()
+        lam = torch.empty(q * N, dtype=torch.float32, device=eng.dev)
+        lam2 = torch.empty_like(lam)
+        dF = torch.zeros(d * N, dtype=torch.float32, device=eng.dev) if eng.ff is not None \
+            else None
+        nw = int(_lib.load().pylbm_hip_adjoint_workspace(eng.ref))
+        part = torch.zeros(max(nw, 2), dtype=torch.float64, device=eng.dev)
+        _lib.call("pylbm_hip_adjoint_seed", eng.ref, _p(ctx.fT), _p(G), _p(lam), _p(dF), _s())
+        seg_buf: list[torch.Tensor] = []
+        for k in range(K - 1, -1, -1):
+            t0 = k * C
+            n = min(C, T - t0)
+            if k == K - 1:
+                states = ctx.last
+            else:
+                while len(seg_buf) < n:
+                    seg_buf.append(eng.state())
+                states = [ctx.ckpts[k]] + seg_buf[1:n]
+                for j in range(1, n):
+                    eng.step(states[j - 1], states[j])
+            for j in range(n - 1, -1, -1):
+                _lib.call("pylbm_hip_adjoint_step", eng.ref, _p(states[j]), _p(lam), _p(lam2),
+                          _p(dF), _p(part), _s())
+                lam, lam2 = lam2, lam
+            if k == K - 1:
+                ctx.last = None
+        need_F, need_tau, need_u, need_rho = ctx.needs
+        du0 = drho0 = None
+        if need_u or need_rho:
+            du = torch.empty((d,) + eng.p.array_shape, dtype=torch.float64, device=eng.dev) \
+                if need_u else None
+            dr = torch.empty(eng.p.array_shape, dtype=torch.float64, device=eng.dev) \
+                if need_rho else None
+            _lib.call("pylbm_hip_adjoint_init", eng.ref, _p(ctx.r0d), _p(ctx.u0d), _p(lam),
+                      _p(du), _p(dr), _s())
+            du0, drho0 = du, dr
+        dtau = None
+        if need_tau:
+            sums = part.view(-1, 2).sum(0)
+            a, b = tau_rate_derivatives(eng.p, eng.tau)
+            dtau = (sums[0] * a + sums[1] * b).reshape(())
+        dFo = dF.view((d,) + eng.p.array_shape) if (dF is not None and need_F) else None
+        ctx.eng = ctx.ckpts = ctx.fT = None
+        return dFo, dtau, du0, drho0, None
+
+
+def rollout(cfg: RolloutConfig, force_field: torch.Tensor | None = None,
+            tau: torch.Tensor | float | None = None, u0: torch.Tensor | None = None,
+            rho0: torch.Tensor | None = None) -> torch.Tensor:
+    """Final post-collision velocity (d, *array_shape), float64, differentiable.
+
+    ``force_field`` (d, *array_shape) must be given exactly when ``cfg.problem`` has a
+    force field; ``tau`` defaults to ``cfg.problem.tau``; ``u0`` / ``rho0`` default to rest.
+    """
+    p = cfg.problem
+    if (force_field is None) != (p.force_field is None):
+        raise ValueError("pass force_field exactly when the problem has a force_field")
+    if tau is None:
+        tau = torch.tensor(p.tau, dtype=torch.float64, device="cuda")
+    elif not torch.is_tensor(tau):
+        tau = torch.tensor(float(tau), dtype=torch.float64, device="cuda")
+    if float(tau) <= 0.5:
+        raise ValueError("tau must be > 0.5")
+    return _RolloutFn.apply(force_field, tau, u0, rho0, cfg)
+
+
+def velocity_loss(u: torch.Tensor, u_target: torch.Tensor) -> torch.Tensor:
+    """0.5 * sum over cells of |u - u_target|^2."""
+    return 0.5 * ((u - u_target) ** 2).sum()

@@ -30,7 +30,7 @@ try:
 except Exception:  # pragma: no cover
     torch = None  # type: ignore[assignment]
 
-_TORCH_DTYPES = {"f32": "float32", "f16": "float16", "bf16": "bfloat16"}
+_TORCH_DTYPES = {"f32": "float32", "f32s": "float32", "f16": "float16", "bf16": "bfloat16"}
 #: the kernels index a plane with 32-bit offsets (one row segment of slack)
 MAX_NODES = 2**31 - 1 - 256
 

@@ -3,9 +3,11 @@
 // caller (in pylbm's hip backend: torch tensors); nothing here allocates.
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 
 #include <string>
 
+#include "lbm_adjoint.hpp"
 #include "lbm_aux.hpp"
 #include "lbm_step.hpp"
 
@@ -51,7 +53,7 @@ int check_params(const pylbm_params* p, const char* what) {
     return fail(PYLBM_EINVAL, "%s: unknown collision %d", what, p->collision);
   if (p->collision == PYLBM_MRT && p->lattice != PYLBM_D2Q9)
     return fail(PYLBM_ENOTSUP, "%s: MRT is implemented for D2Q9 only", what);
-  if (p->storage < PYLBM_F32 || p->storage > PYLBM_BF16)
+  if (p->storage < PYLBM_F32 || p->storage > PYLBM_F32S)
     return fail(PYLBM_EINVAL, "%s: unknown storage %d", what, p->storage);
   if (p->aa != 0 && p->aa != 1) return fail(PYLBM_EINVAL, "%s: aa must be 0 or 1", what);
   if (p->nx < 1 || p->ny < 1 || p->nz < 1)
@@ -74,6 +76,20 @@ int check_params(const pylbm_params* p, const char* what) {
   return 0;
 }
 
+// Nodes per lane of the fused step: the default per storage type, or PYLBM_HIP_VEC
+// (1 = one node per lane; fp32 2 or 4; 16-bit 4 or 8) to compare the kernels.
+int vec_choice(int storage, int nx) {
+  static const int forced = [] {
+    const char* e = getenv("PYLBM_HIP_VEC");
+    return e ? atoi(e) : 0;
+  }();
+  const bool wide = storage == PYLBM_F32 || storage == PYLBM_F32S;
+  const int lo = wide ? 2 : 4, hi = wide ? 4 : 8;
+  int v = 4;
+  if (forced == 1 || forced == lo || forced == hi) v = forced;
+  return (v > 1 && nx % v == 0) ? v : 1;
+}
+
 int make_kargs(const pylbm_params* p, KArgs& a, const char* what) {
   a.nx = p->nx;
   a.ny = p->ny;
@@ -88,6 +104,8 @@ int make_kargs(const pylbm_params* p, KArgs& a, const char* what) {
   for (int k = 0; k < dims; ++k)
     if (p->periodic[k]) a.per |= 1 << k;
   a.remap = (a.nblk % 8 == 0) ? 1 : 0;
+  // vectorised kernel whenever rows split into whole vectors
+  a.vec = vec_choice(p->storage, p->nx);
   a.ulid = p->ulid;
   a.fx = p->force[0];
   a.fy = p->force[1];
@@ -240,6 +258,61 @@ int pylbm_hip_copy(const void* src, void* dst, int64_t nbytes, void* stream) {
     return fail(PYLBM_EINVAL, "pylbm_hip_copy: nbytes must be a non-negative multiple of 16");
   if (nbytes == 0) return 0;
   return hip_fail(launch_copy(src, dst, nbytes, (hipStream_t)stream), "pylbm_hip_copy");
+}
+
+int64_t pylbm_hip_adjoint_workspace(const pylbm_params* p) {
+  if (check_params(p, "pylbm_hip_adjoint_workspace")) return -1;
+  KArgs a;
+  if (make_kargs(p, a, "pylbm_hip_adjoint_workspace")) return -1;
+  return adjoint_partials(a);
+}
+
+static int check_adjoint(const pylbm_params* p, const char* what) {
+  if (int e = check_params(p, what)) return e;
+  if (p->aa) return fail(PYLBM_EINVAL, "%s: the adjoint runs on the two-buffer layout", what);
+  if (p->collision == PYLBM_MRT)
+    return fail(PYLBM_ENOTSUP, "%s: the adjoint covers BGK and TRT", what);
+  return 0;
+}
+
+int pylbm_hip_adjoint_seed(const pylbm_params* p, const void* f_T, const float* grad_u,
+                           float* lam, float* dF, void* stream) {
+  if (int e = check_adjoint(p, "pylbm_hip_adjoint_seed")) return e;
+  if (!f_T || !grad_u || !lam) return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_seed: NULL buffer");
+  if (p->force_field && !dF)
+    return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_seed: a force field needs dF");
+  KArgs a;
+  if (int e = make_kargs(p, a, "pylbm_hip_adjoint_seed")) return e;
+  return hip_fail(launch_adjoint_seed(p->lattice, a, p->storage, p->force_field != nullptr,
+                                      f_T, grad_u, lam, dF, (hipStream_t)stream),
+                  "pylbm_hip_adjoint_seed");
+}
+
+int pylbm_hip_adjoint_step(const pylbm_params* p, const void* f_t, const float* lam_in,
+                           float* lam_out, float* dF, double* dtau_part, void* stream) {
+  if (int e = check_adjoint(p, "pylbm_hip_adjoint_step")) return e;
+  if (!f_t || !lam_in || !lam_out || !dtau_part)
+    return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_step: NULL buffer");
+  if (lam_in == lam_out)
+    return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_step: lam_in and lam_out must differ");
+  if (p->force_field && !dF)
+    return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_step: a force field needs dF");
+  KArgs a;
+  if (int e = make_kargs(p, a, "pylbm_hip_adjoint_step")) return e;
+  return hip_fail(launch_adjoint_step(p->lattice, a, p->collision, p->storage,
+                                      p->force_field != nullptr, f_t, lam_in, lam_out, dF,
+                                      dtau_part, (hipStream_t)stream),
+                  "pylbm_hip_adjoint_step");
+}
+
+int pylbm_hip_adjoint_init(const pylbm_params* p, const double* rho0, const double* u0,
+                           const float* lam0, double* du0, double* drho0, void* stream) {
+  if (int e = check_params(p, "pylbm_hip_adjoint_init")) return e;
+  if (!lam0) return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_init: NULL lam0");
+  if (!du0 && !drho0) return 0;
+  return hip_fail(launch_adjoint_init(p->lattice, (int)p->n_nodes, rho0, u0, lam0, du0, drho0,
+                                      (hipStream_t)stream),
+                  "pylbm_hip_adjoint_init");
 }
 
 }  // extern "C"

@@ -1,0 +1,371 @@
+// Discrete adjoint of the fused two-buffer step (BGK / TRT with Guo forcing), for gradients
+// of a loss of the final velocity field with respect to a per-cell Guo force F(x), tau and
+// the initial velocity.  The reference has no gradient path; the float64 cross-checks are a
+// torch-autograd restatement of the same step and central finite differences.
+//
+// Forward step at node x (pull, as in the step kernels):
+//   g_i(x) = f_i(x - c_i), or f_opp(i)(x) + lid_i on a link crossing a wall
+//   p(x)   = g - s+ P+ (g - feq(g)) - s- P- (g - feq(g)) + (1 - s+/2) P+ S + (1 - s-/2) P- S
+// with P+- the (anti)symmetric projectors over opposite pairs (BGK: s+ = s- = omega), rho,
+// u = (sum c g + F/2)/rho, feq_i = w_i rho (1 + 3 cu + 4.5 cu^2 - 1.5 u^2) and
+// S_i = w_i (3 (c_i - u).F + 9 (c_i.u)(c_i.F)).
+//
+// Adjoint of one step, per node x, for lam = dL/dp(x):
+//   lam'  = (s+ P+ + s- P-) lam,   lam'' = (g+ P+ + g- P-) lam,   g+- = 1 - s+-/2
+//   A     = sum_i lam'_i feq_i / rho
+//   B_a   = sum_i lam'_i dfeq_i/du_a + lam''_i dS_i/du_a
+//   mu_k  = lam_k - lam'_k + A + sum_a B_a (c_ka - u_a) / rho          (= J^T lam)
+// and mu_k is pushed back along the link it was pulled over: to (k, x - c_k), or to
+// (opp(k), x) on a wall link.  Pull streaming with bounce-back is a permutation, so every
+// slot of the previous adjoint state is written exactly once (no atomics).  The lid term is
+// a constant and drops out.  Per-cell force and rate sensitivities:
+//   dL/dF_b(x) += B_b / (2 rho) + sum_i lam''_i w_i (3 (c_ib - u_b) + 9 cu_i c_ib)
+//   dL/ds+-    += -sum_i lam+-_i (g_i - feq_i + S_i / 2)      (lam+- = P+- lam)
+#include "lbm_adjoint.hpp"
+
+namespace plbm {
+
+namespace {
+
+// Per-node forward quantities and the transposed collision Jacobian.  h_i = g_i - w_i is
+// the deviation of the pulled population from its rest weight (exact in fp32 for g near
+// w), and every difference the Jacobian needs (g - feq, feq - w) is formed from
+// deviations, so the small non-equilibrium part is not lost to cancellation.
+template <class L, int COLL>
+__device__ __forceinline__ void adjoint_node(const float (&h)[L::Q], const float F[3],
+                                             const float (&lam)[L::Q], float sp, float sm,
+                                             float (&mu)[L::Q], float dF[3], float& dsp,
+                                             float& dsm) {
+  constexpr int Q = L::Q, D = L::D;
+  float drho = 0.f, j[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    drho += h[i];
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+      if (L::c(i, a)) j[a] += L::c(i, a) * h[i];
+  }
+  const float rho = 1.0f + drho;
+  const float inv = 1.0f / rho;
+  float u[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int a = 0; a < D; ++a) u[a] = (j[a] + 0.5f * F[a]) * inv;
+  const float usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  const float uf = u[0] * F[0] + u[1] * F[1] + u[2] * F[2];
+  const float gp = 1.0f - 0.5f * sp, gm = 1.0f - 0.5f * sm;
+
+  float A = 0.f, B[3] = {0.f, 0.f, 0.f}, dFs[3] = {0.f, 0.f, 0.f};
+  dsp = 0.f;
+  dsm = 0.f;
+  float lp_[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const int o = L::opp(i);
+    const float lplus = 0.5f * (lam[i] + lam[o]);
+    const float lminus = 0.5f * (lam[i] - lam[o]);
+    const float lp = (COLL == PYLBM_BGK) ? sp * lam[i] : sp * lplus + sm * lminus;
+    const float lpp = (COLL == PYLBM_BGK) ? gp * lam[i] : gp * lplus + gm * lminus;
+    lp_[i] = lp;
+    const float w = (float)L::w(i);
+    const float cu = L::c(i, 0) * u[0] + L::c(i, 1) * u[1] + L::c(i, 2) * u[2];
+    const float cf = L::c(i, 0) * F[0] + L::c(i, 1) * F[1] + L::c(i, 2) * F[2];
+    const float poly = cu * (3.0f + 4.5f * cu) - 1.5f * usq;   // feq = w rho (1 + poly)
+    const float feq_dev = w * (drho + rho * poly);             // feq - w
+    const float S = w * (3.0f * (cf - uf) + 9.0f * cu * cf);
+    A += lp * (w + feq_dev);
+    const float wr = w * rho;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      const float ca = (float)L::c(i, a);
+      // dfeq_i/du_a = w rho (3 c_a + 9 cu c_a - 3 u_a);  dS_i/du_a = w (-3 F_a + 9 c_a cf)
+      B[a] += lp * wr * (3.0f * ca + 9.0f * cu * ca - 3.0f * u[a]) +
+              lpp * w * (-3.0f * F[a] + 9.0f * ca * cf);
+      // dS_i/dF_a at fixed u = w (3 (c_a - u_a) + 9 cu c_a)
+      dFs[a] += lpp * w * (3.0f * (ca - u[a]) + 9.0f * cu * ca);
+    }
+    const float r = (h[i] - feq_dev) + 0.5f * S;  // g - feq + S/2
+    if (COLL == PYLBM_BGK) {
+      dsp -= lam[i] * r;
+    } else {
+      dsp -= lplus * r;
+      dsm -= lminus * r;
+    }
+  }
+  A *= inv;
+#pragma unroll
+  for (int k = 0; k < Q; ++k) {
+    float m = lam[k] - lp_[k] + A;
+#pragma unroll
+    for (int a = 0; a < D; ++a) m += B[a] * ((float)L::c(k, a) - u[a]) * inv;
+    mu[k] = m;
+  }
+#pragma unroll
+  for (int a = 0; a < D; ++a) dF[a] = 0.5f * B[a] * inv + dFs[a];
+}
+
+__device__ __forceinline__ float wave_sum_f(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+template <class L, int COLL, int ST, bool FF>
+__global__ void __launch_bounds__(kBlock)
+k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f,
+          const float* __restrict__ lam_in, float* __restrict__ lam_out,
+          float* __restrict__ dF, double* __restrict__ part) {
+  using C = Codec<ST>;
+  constexpr int Q = L::Q;
+  const int b = (int)blockIdx.x;
+  const int seg = b % a.nseg;
+  const int row = b / a.nseg;
+  const int y = row % a.ny;
+  const int z = row / a.ny;
+  const int x = seg * kBlock + (int)threadIdx.x;
+  const bool live = x < a.nx;
+  float dsp = 0.f, dsm = 0.f;
+  if (live) {
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int n = x + nx * (y + ny * z);
+    const size_t N = (size_t)a.N;
+    const bool per[3] = {(a.per & 1) != 0, ((a.per >> 1) & 1) != 0, ((a.per >> 2) & 1) != 0};
+    const int dims[3] = {nx, ny, nz};
+    const int crd[3] = {x, y, z};
+    float g[Q], lam[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      int s[3] = {x, y, z};
+      bool wall = false, top = false;
+#pragma unroll
+      for (int d = 0; d < L::D; ++d) {
+        const int c = L::c(i, d);
+        if (c == 0) continue;
+        s[d] = crd[d] - c;
+        if (s[d] < 0 || s[d] >= dims[d]) {
+          if (per[d]) s[d] += (s[d] < 0) ? dims[d] : -dims[d];
+          else { wall = true; if (d == L::D - 1 && s[d] >= dims[d]) top = true; }
+        }
+      }
+      const int src = s[0] + nx * (s[1] + ny * s[2]);
+      const size_t idx = wall ? (size_t)L::opp(i) * N + n : (size_t)i * N + src;
+      // deviation h = g - w: the shifted storages hold it, fp32 subtracts exactly
+      float v = C::shifted ? C::ld(f[idx]) : C::ld(f[idx]) - (float)L::w(i);
+      if (L::c(i, 0) != 0 && top) v += (float)(6.0 * L::w(i) * L::c(i, 0)) * a.ulid;
+      g[i] = v;
+      lam[i] = lam_in[(size_t)i * N + n];
+    }
+    float F[3] = {a.fx, a.fy, a.fz};
+    if (FF) {
+#pragma unroll
+      for (int d = 0; d < L::D; ++d) F[d] = a.ff[(size_t)d * N + n];
+    }
+    float mu[Q], dFn[3];
+    adjoint_node<L, COLL>(g, F, lam, a.r0, a.r1, mu, dFn, dsp, dsm);
+    // push mu_k back along the link it was pulled over
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      int s[3] = {x, y, z};
+      bool wall = false;
+#pragma unroll
+      for (int d = 0; d < L::D; ++d) {
+        const int c = L::c(i, d);
+        if (c == 0) continue;
+        s[d] = crd[d] - c;
+        if (s[d] < 0 || s[d] >= dims[d]) {
+          if (per[d]) s[d] += (s[d] < 0) ? dims[d] : -dims[d];
+          else wall = true;
+        }
+      }
+      const size_t idx = wall ? (size_t)L::opp(i) * N + n
+                              : (size_t)i * N + (s[0] + nx * (s[1] + ny * s[2]));
+      lam_out[idx] = mu[i];
+    }
+    if (FF) {
+#pragma unroll
+      for (int d = 0; d < L::D; ++d) dF[(size_t)d * N + n] += dFn[d];
+    }
+  }
+  // per-workgroup partial sums of dL/ds+ and dL/ds- (deterministic: one slot per group)
+  __shared__ float red[2][kBlock / 64];
+  dsp = wave_sum_f(dsp);
+  dsm = wave_sum_f(dsm);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    red[0][wv] = dsp;
+    red[1][wv] = dsm;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s0 = 0.0, s1 = 0.0;
+    for (int k = 0; k < kBlock / 64; ++k) {
+      s0 += red[0][k];
+      s1 += red[1][k];
+    }
+    part[2 * (size_t)b] += s0;
+    part[2 * (size_t)b + 1] += s1;
+  }
+}
+
+// Terminal adjoint state of L(u(f)), u = (sum c f - F/2)/rho post-collision:
+//   lam_k = sum_a G_a (c_ka - u_a) / rho,   dF_a = -G_a / (2 rho)   (G = dL/du)
+template <class L, int ST, bool FF>
+__global__ void __launch_bounds__(kBlock)
+k_adjoint_seed(const KArgs a, const typename Codec<ST>::T* __restrict__ f,
+               const float* __restrict__ G, float* __restrict__ lam, float* __restrict__ dF) {
+  using C = Codec<ST>;
+  const int n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= a.N) return;
+  const size_t N = (size_t)a.N;
+  float fv[L::Q], rho = 0.f, j[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int i = 0; i < L::Q; ++i) {
+    fv[i] = C::ld(f[(size_t)i * N + n]) + (C::shifted ? (float)L::w(i) : 0.0f);
+    rho += fv[i];
+#pragma unroll
+    for (int d = 0; d < L::D; ++d) j[d] += L::c(i, d) * fv[i];
+  }
+  float F[3] = {a.fx, a.fy, a.fz};
+  if (FF) {
+#pragma unroll
+    for (int d = 0; d < L::D; ++d) F[d] = a.ff[(size_t)d * N + n];
+  }
+  const float inv = 1.0f / rho;
+  float u[3] = {0.f, 0.f, 0.f}, Gv[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int d = 0; d < L::D; ++d) {
+    u[d] = (j[d] - 0.5f * F[d]) * inv;
+    Gv[d] = G[(size_t)d * N + n];
+  }
+#pragma unroll
+  for (int k = 0; k < L::Q; ++k) {
+    float m = 0.f;
+#pragma unroll
+    for (int d = 0; d < L::D; ++d) m += Gv[d] * ((float)L::c(k, d) - u[d]);
+    lam[(size_t)k * N + n] = m * inv;
+  }
+  if (FF) {
+#pragma unroll
+    for (int d = 0; d < L::D; ++d) dF[(size_t)d * N + n] = -0.5f * Gv[d] * inv;
+  }
+}
+
+// Gradient of the initial equilibrium f0 = feq(rho0, u0) (float64 fields) with respect to
+// u0 and rho0, given lam0 = dL/df0.
+template <class L>
+__global__ void __launch_bounds__(kBlock)
+k_adjoint_init(int N_, const double* __restrict__ rho0, const double* __restrict__ u0,
+               const float* __restrict__ lam, double* __restrict__ du, double* __restrict__ drho) {
+  const int n = blockIdx.x * kBlock + threadIdx.x;
+  if (n >= N_) return;
+  const size_t N = (size_t)N_;
+  const double r = rho0 ? rho0[n] : 1.0;
+  double u[3] = {0.0, 0.0, 0.0};
+  if (u0) {
+#pragma unroll
+    for (int d = 0; d < L::D; ++d) u[d] = u0[(size_t)d * N + n];
+  }
+  const double usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  double gu[3] = {0.0, 0.0, 0.0}, gr = 0.0;
+#pragma unroll
+  for (int i = 0; i < L::Q; ++i) {
+    const double l = (double)lam[(size_t)i * N + n];
+    const double cu = L::c(i, 0) * u[0] + L::c(i, 1) * u[1] + L::c(i, 2) * u[2];
+    gr += l * L::w(i) * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * usq);
+#pragma unroll
+    for (int d = 0; d < L::D; ++d)
+      gu[d] += l * L::w(i) * r * (3.0 * L::c(i, d) + 9.0 * cu * L::c(i, d) - 3.0 * u[d]);
+  }
+  if (du) {
+#pragma unroll
+    for (int d = 0; d < L::D; ++d) du[(size_t)d * N + n] = gu[d];
+  }
+  if (drho) drho[n] = gr;
+}
+
+template <class L, int COLL, int ST>
+int adj_step_t(const KArgs& a, bool field, const void* f, const float* lin, float* lout,
+               float* dF, double* part, hipStream_t s) {
+  using T = typename Codec<ST>::T;
+  const dim3 grid((unsigned)a.nblk), block(kBlock);
+  if (field)
+    k_adjoint<L, COLL, ST, true><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF, part);
+  else
+    k_adjoint<L, COLL, ST, false><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF, part);
+  return (int)hipGetLastError();
+}
+
+template <class L, int COLL>
+int adj_step_s(const KArgs& a, int storage, bool field, const void* f, const float* lin,
+               float* lout, float* dF, double* part, hipStream_t s) {
+  switch (storage) {
+    case PYLBM_F32: return adj_step_t<L, COLL, PYLBM_F32>(a, field, f, lin, lout, dF, part, s);
+    case PYLBM_F16: return adj_step_t<L, COLL, PYLBM_F16>(a, field, f, lin, lout, dF, part, s);
+    case PYLBM_BF16: return adj_step_t<L, COLL, PYLBM_BF16>(a, field, f, lin, lout, dF, part, s);
+    case PYLBM_F32S: return adj_step_t<L, COLL, PYLBM_F32S>(a, field, f, lin, lout, dF, part, s);
+  }
+  return PYLBM_EINVAL;
+}
+
+template <class L>
+int adj_step_l(const KArgs& a, int coll, int storage, bool field, const void* f,
+               const float* lin, float* lout, float* dF, double* part, hipStream_t s) {
+  if (coll == PYLBM_BGK)
+    return adj_step_s<L, PYLBM_BGK>(a, storage, field, f, lin, lout, dF, part, s);
+  if (coll == PYLBM_TRT)
+    return adj_step_s<L, PYLBM_TRT>(a, storage, field, f, lin, lout, dF, part, s);
+  return PYLBM_ENOTSUP;
+}
+
+template <class L, int ST>
+int seed_t(const KArgs& a, bool field, const void* f, const float* G, float* lam, float* dF,
+           hipStream_t s) {
+  using T = typename Codec<ST>::T;
+  const dim3 grid((unsigned)((a.N + kBlock - 1) / kBlock)), block(kBlock);
+  if (field)
+    k_adjoint_seed<L, ST, true><<<grid, block, 0, s>>>(a, (const T*)f, G, lam, dF);
+  else
+    k_adjoint_seed<L, ST, false><<<grid, block, 0, s>>>(a, (const T*)f, G, lam, dF);
+  return (int)hipGetLastError();
+}
+
+template <class L>
+int seed_l(const KArgs& a, int storage, bool field, const void* f, const float* G, float* lam,
+           float* dF, hipStream_t s) {
+  switch (storage) {
+    case PYLBM_F32: return seed_t<L, PYLBM_F32>(a, field, f, G, lam, dF, s);
+    case PYLBM_F16: return seed_t<L, PYLBM_F16>(a, field, f, G, lam, dF, s);
+    case PYLBM_BF16: return seed_t<L, PYLBM_BF16>(a, field, f, G, lam, dF, s);
+    case PYLBM_F32S: return seed_t<L, PYLBM_F32S>(a, field, f, G, lam, dF, s);
+  }
+  return PYLBM_EINVAL;
+}
+
+}  // namespace
+
+int adjoint_partials(const KArgs& a) { return 2 * a.nblk; }
+
+int launch_adjoint_step(int lattice, const KArgs& a, int coll, int storage, bool field,
+                        const void* f, const float* lin, float* lout, float* dF, double* part,
+                        hipStream_t s) {
+  if (lattice == PYLBM_D2Q9)
+    return adj_step_l<D2Q9>(a, coll, storage, field, f, lin, lout, dF, part, s);
+  return adj_step_l<D3Q19>(a, coll, storage, field, f, lin, lout, dF, part, s);
+}
+
+int launch_adjoint_seed(int lattice, const KArgs& a, int storage, bool field, const void* f,
+                        const float* G, float* lam, float* dF, hipStream_t s) {
+  if (lattice == PYLBM_D2Q9) return seed_l<D2Q9>(a, storage, field, f, G, lam, dF, s);
+  return seed_l<D3Q19>(a, storage, field, f, G, lam, dF, s);
+}
+
+int launch_adjoint_init(int lattice, int N, const double* rho0, const double* u0,
+                        const float* lam, double* du, double* drho, hipStream_t s) {
+  const dim3 grid((unsigned)((N + kBlock - 1) / kBlock)), block(kBlock);
+  if (lattice == PYLBM_D2Q9)
+    k_adjoint_init<D2Q9><<<grid, block, 0, s>>>(N, rho0, u0, lam, du, drho);
+  else
+    k_adjoint_init<D3Q19><<<grid, block, 0, s>>>(N, rho0, u0, lam, du, drho);
+  return (int)hipGetLastError();
+}
+
+}  // namespace plbm

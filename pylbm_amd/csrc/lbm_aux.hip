@@ -286,6 +286,7 @@ int aux_lat(int op, const AuxArgs& x, hipStream_t s) {
     case PYLBM_F32: return aux_typed<L, PYLBM_F32>(op, x, s);
     case PYLBM_F16: return aux_typed<L, PYLBM_F16>(op, x, s);
     case PYLBM_BF16: return aux_typed<L, PYLBM_BF16>(op, x, s);
+    case PYLBM_F32S: return aux_typed<L, PYLBM_F32S>(op, x, s);
   }
   return PYLBM_EINVAL;
 }

@@ -73,6 +73,15 @@ template <> struct Codec<PYLBM_F32> {
   __device__ static __forceinline__ T st(float v) { return v; }
 };
 
+// fp32 holding the deviation f - w_i: same bytes as PYLBM_F32, but the small part of f that
+// the collision relaxes keeps ~7 significant digits (used by the differentiable rollouts).
+template <> struct Codec<PYLBM_F32S> {
+  using T = float;
+  static constexpr bool shifted = true;
+  __device__ static __forceinline__ float ld(T v) { return v; }
+  __device__ static __forceinline__ T st(float v) { return v; }
+};
+
 template <> struct Codec<PYLBM_F16> {
   using T = _Float16;
   static constexpr bool shifted = true;
@@ -94,6 +103,10 @@ template <> struct Codec<PYLBM_BF16> {
   }
 };
 
+__device__ __forceinline__ float Codec_ld(float v) { return v; }
+__device__ __forceinline__ float Codec_ld(_Float16 v) { return (float)v; }
+__device__ __forceinline__ float Codec_ld(uint16_t v) { return __uint_as_float(((uint32_t)v) << 16); }
+
 // ------------------------------------------------------------------------ kernel args
 struct KArgs {
   int nx, ny, nz;
@@ -102,6 +115,7 @@ struct KArgs {
   int nblk;    // total workgroups
   int per;     // bit a: axis a periodic
   int remap;   // 1: XCD-aware workgroup -> row-segment mapping
+  int vec;     // >1: use the vectorised kernel (nodes per lane for its storage type)
   float ulid;
   float fx, fy, fz;
   float r0, r1, r2, r3;
@@ -118,50 +132,56 @@ __device__ __forceinline__ int remap_block(int b, int nblk, int remap) {
 }
 
 // ------------------------------------------------------------------------ collision
-// Per-node collision in fp32.  f: post-streaming populations; F: Guo force at the node
-// (zero when the problem has no force, which leaves every term exact); returns p.
+// Per-node collision in fp32, written on deviations from the rest weights:
+// h_i = f_i - w_i in, p_i - w_i out.  With rho = 1 + sum h, u = (sum c h + F/2) / rho and
+// feq_i - w_i = w_i (drho + rho (3 cu + 4.5 cu^2 - 1.5 u^2)), every operator below is the
+// reference's (pylbm/core.py:203-247) with f and feq replaced by their deviations (the
+// weights cancel in f - feq and sum_i c_i w_i = 0), so the small non-equilibrium part is
+// never formed as a difference of two O(w) numbers.  F: Guo force at the node (zero when
+// the problem has none, which leaves every term exact).
 template <class L, int COLL>
-__device__ __forceinline__ void collide_node(const float (&f)[L::Q], float (&p)[L::Q],
+__device__ __forceinline__ void collide_node(const float (&h)[L::Q], float (&d)[L::Q],
                                              const float F[3], bool force, float r0, float r1,
                                              float r2, float r3) {
   constexpr int Q = L::Q;
-  float rho = 0.f, j0 = 0.f, j1 = 0.f, j2 = 0.f;
+  float drho = 0.f, j0 = 0.f, j1 = 0.f, j2 = 0.f;
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
-    rho += f[i];
-    if (L::c(i, 0)) j0 += L::c(i, 0) * f[i];
-    if (L::c(i, 1)) j1 += L::c(i, 1) * f[i];
-    if (L::D == 3 && L::c(i, 2)) j2 += L::c(i, 2) * f[i];
+    drho += h[i];
+    if (L::c(i, 0)) j0 += L::c(i, 0) * h[i];
+    if (L::c(i, 1)) j1 += L::c(i, 1) * h[i];
+    if (L::D == 3 && L::c(i, 2)) j2 += L::c(i, 2) * h[i];
   }
+  const float rho = 1.0f + drho;
   const float inv = 1.0f / rho;
   const float ux = (j0 + 0.5f * F[0]) * inv;
   const float uy = (j1 + 0.5f * F[1]) * inv;
   const float uz = (L::D == 3) ? (j2 + 0.5f * F[2]) * inv : 0.0f;
   const float usq = ux * ux + uy * uy + uz * uz;
-  const float base = 1.0f - 1.5f * usq;
   const float uf = ux * F[0] + uy * F[1] + uz * F[2];
-  float feq[Q], S[Q];
+  float neq[Q], S[Q];  // neq = f - feq = h - (feq - w)
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     const float cu = L::c(i, 0) * ux + L::c(i, 1) * uy + L::c(i, 2) * uz;
     const float w = (float)L::w(i);
-    feq[i] = w * rho * (base + cu * (3.0f + 4.5f * cu));
+    const float poly = cu * (3.0f + 4.5f * cu) - 1.5f * usq;
+    neq[i] = h[i] - w * (drho + rho * poly);
     const float cf = L::c(i, 0) * F[0] + L::c(i, 1) * F[1] + L::c(i, 2) * F[2];
     S[i] = force ? w * (3.0f * (cf - uf) + 9.0f * cu * cf) : 0.0f;
   }
   if constexpr (COLL == PYLBM_BGK) {
     const float g = 1.0f - 0.5f * r0;
 #pragma unroll
-    for (int i = 0; i < Q; ++i) p[i] = f[i] + r0 * (feq[i] - f[i]) + g * S[i];
+    for (int i = 0; i < Q; ++i) d[i] = h[i] - r0 * neq[i] + g * S[i];
   } else if constexpr (COLL == PYLBM_TRT) {
     const float gp = 1.0f - 0.5f * r0, gm = 1.0f - 0.5f * r1;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       const int o = L::opp(i);
-      const float np_ = 0.5f * (f[i] + f[o]) - 0.5f * (feq[i] + feq[o]);
-      const float nm_ = 0.5f * (f[i] - f[o]) - 0.5f * (feq[i] - feq[o]);
+      const float np_ = 0.5f * (neq[i] + neq[o]);
+      const float nm_ = 0.5f * (neq[i] - neq[o]);
       const float sp_ = 0.5f * (S[i] + S[o]), sm_ = 0.5f * (S[i] - S[o]);
-      p[i] = f[i] - r0 * np_ - r1 * nm_ + gp * sp_ + gm * sm_;
+      d[i] = h[i] - r0 * np_ - r1 * nm_ + gp * sp_ + gm * sm_;
     }
   } else {
     static_assert(COLL != PYLBM_MRT || L::Q == 9, "MRT is D2Q9 only");
@@ -173,18 +193,18 @@ __device__ __forceinline__ void collide_node(const float (&f)[L::Q], float (&p)[
 #pragma unroll
       for (int i = 0; i < 9; ++i) {
         if (mrt_m(k, i) == 0) continue;
-        mne += (float)mrt_m(k, i) * (f[i] - feq[i]);
+        mne += (float)mrt_m(k, i) * neq[i];
         ms += (float)mrt_m(k, i) * S[i];
       }
       dm[k] = (-s[k] * mne + (1.0f - 0.5f * s[k]) * ms) / (float)mrt_norm(k);
     }
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
-      float acc = f[i];
+      float acc = h[i];
 #pragma unroll
       for (int k = 0; k < 9; ++k)
         if (mrt_m(k, i) != 0) acc += (float)mrt_m(k, i) * dm[k];
-      p[i] = acc;
+      d[i] = acc;
     }
   }
 }

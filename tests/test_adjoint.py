@@ -1,0 +1,230 @@
+"""Gradients of the checkpointed rollout (HIP adjoint) vs float64 references.
+
+There is no reference gradient path (the reference's torch rungs run under no_grad), so
+the adjoint is pinned by: (a) float64 torch autograd through oracle/torch_ref.py, a
+restatement of the step that tests below tie to the NumPy oracle; (b) central finite
+differences of the float64 C oracle; (c) a per-step VJP comparison on random states.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import lbm_ref as R
+from oracle import lbm_ref_c as RC
+from oracle.torch_ref import TorchLBM
+from pylbm_amd.adjoint import tau_rate_derivatives
+from pylbm_amd.fields import divergence_free_field, smooth_random_field
+from pylbm_amd.problem import Problem
+
+# ----------------------------------------------------------------------------- CPU
+
+
+@pytest.mark.parametrize("kw", [
+    dict(shape=(9, 7), tau=0.8, periodic=(True, False), collision="bgk"),
+    dict(shape=(9, 7), tau=0.7, u_lid=0.05, collision="trt"),
+    dict(shape=(5, 4, 6), tau=0.75, periodic=(True, False, True), collision="trt"),
+])
+def test_torch_restatement_equals_oracle(kw):
+    rng = np.random.default_rng(0)
+    case = R.Case(**kw)
+    ff = 1e-4 * rng.standard_normal((case.dim,) + case.array_shape)
+    case_f = R.Case(**kw, force_field=ff)
+    f0 = R.initial_populations(case) * (1 + 0.02 * rng.standard_normal(
+        (case.lat.q,) + case.array_shape))
+    tl = TorchLBM(case_f)
+    got = tl.rollout(torch.tensor(f0), torch.tensor(ff), torch.tensor(case.tau, dtype=torch.float64),
+                     5).numpy()
+    assert np.abs(got - R.run(case_f, 5, f0)).max() < 1e-15
+
+
+def test_torch_restatement_gradient_matches_finite_differences():
+    case = R.Case((8, 6), tau=0.8, periodic=(True, False), collision="trt")
+    rng = np.random.default_rng(1)
+    ff = torch.tensor(1e-5 + 1e-6 * rng.standard_normal((2, 6, 8)), requires_grad=True)
+    tau = torch.tensor(0.8, dtype=torch.float64, requires_grad=True)
+    tl = TorchLBM(case)
+    ut = torch.tensor(0.001 * rng.standard_normal((2, 6, 8)))
+
+    def loss(F, t):
+        f = tl.rollout(tl.equilibrium(torch.ones(6, 8, dtype=torch.float64),
+                                      torch.zeros(2, 6, 8, dtype=torch.float64)), F, t, 30)
+        return 0.5 * ((tl.velocity(f, F) - ut) ** 2).sum()
+
+    L = loss(ff, tau)
+    gF, gt = torch.autograd.grad(L, (ff, tau))
+    h = 1e-4 * 0.8
+    fd = (loss(ff.detach(), tau.detach() + h) - loss(ff.detach(), tau.detach() - h)) / (2 * h)
+    assert abs(float(gt) - float(fd)) < 1e-6 * abs(float(fd))
+    V = torch.tensor(rng.standard_normal((2, 6, 8)))
+    hF = 1e-4 * 1e-5
+    fdF = (loss(ff.detach() + hF * V, tau.detach()) - loss(ff.detach() - hF * V, tau.detach())) \
+        / (2 * hF)
+    assert abs(float((gF * V).sum()) - float(fdF)) < 1e-6 * abs(float(fdF))
+
+
+@pytest.mark.parametrize("coll", ["bgk", "trt"])
+def test_tau_rate_derivatives(coll):
+    p = Problem((4, 4), tau=0.8, collision=coll)
+    a, b = tau_rate_derivatives(p, 0.8)
+    h = 1e-6
+
+    def rates(t):
+        return Problem((4, 4), tau=t, collision=coll).runtime_rates()
+
+    fd = [(rp - rm) / (2 * h) for rp, rm in zip(rates(0.8 + h), rates(0.8 - h))]
+    assert np.isclose(a, fd[0], rtol=1e-7)
+    assert np.isclose(b, fd[1] if coll == "trt" else 0.0, rtol=1e-7, atol=1e-12)
+
+
+# ----------------------------------------------------------------------------- GPU
+
+
+def _dev(a):
+    return torch.as_tensor(a, device="cuda")
+
+
+VJP_CASES = {
+    "d2q9-bgk-channel": dict(shape=(24, 16), tau=0.8, periodic=(True, False), collision="bgk"),
+    "d2q9-trt-cavity": dict(shape=(20, 12), tau=0.7, u_lid=0.05, collision="trt"),
+    "d3q19-bgk-periodic": dict(shape=(12, 8, 6), tau=0.7, periodic=(True, True, True)),
+    "d3q19-trt-walls": dict(shape=(10, 8, 6), tau=0.65, periodic=(True, False, False),
+                            collision="trt", u_lid=0.03),
+}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("storage", ["f32", "f32s"])
+@pytest.mark.parametrize("name", list(VJP_CASES))
+def test_adjoint_step_equals_float64_vjp(name, storage):
+    """One adjoint step (lam, dF, dL/dtau) == torch float64 VJP of one restated step."""
+    from pylbm_amd import _lib
+    from pylbm_amd.adjoint import RolloutConfig, _Engine
+
+    kw = VJP_CASES[name]
+    rng = np.random.default_rng(5)
+    case = R.Case(**kw)
+    q, d, shp = case.lat.q, case.dim, case.array_shape
+    ff = (1e-4 * rng.standard_normal((d,) + shp)).astype(np.float32).astype(np.float64)
+    f = (R.initial_populations(case) * (1 + 0.02 * rng.standard_normal((q,) + shp)))
+    f = f.astype(np.float32).astype(np.float64)
+    lam = rng.standard_normal((q,) + shp).astype(np.float32).astype(np.float64)
+    # float64 reference
+    tl = TorchLBM(R.Case(**kw, force_field=ff))
+    ft = torch.tensor(f, requires_grad=True)
+    Ft = torch.tensor(ff, requires_grad=True)
+    tt = torch.tensor(case.tau, dtype=torch.float64, requires_grad=True)
+    out = tl.step(ft, Ft, tt)
+    gf, gF, gt = torch.autograd.grad((out * torch.tensor(lam)).sum(), (ft, Ft, tt))
+    # HIP
+    p = Problem(**kw, force_field=ff)
+    eng = _Engine(RolloutConfig(p, 1, storage=storage),
+                  _dev(ff.astype(np.float32)).reshape(d, -1), case.tau)
+    stored = R.encode(f, storage, False, case.lat)  # f32: f; f32s: f - w, both in fp32
+    fs = _dev(stored).reshape(-1)
+    lin = _dev(lam.astype(np.float32)).reshape(-1)
+    lout = torch.full_like(lin, float("nan"))
+    dF = torch.zeros(d * p.n_nodes, dtype=torch.float32, device="cuda")
+    part = torch.zeros(int(_lib.load().pylbm_hip_adjoint_workspace(eng.ref)), dtype=torch.float64,
+                       device="cuda")
+    _lib.call("pylbm_hip_adjoint_step", eng.ref, fs.data_ptr(), lin.data_ptr(),
+              lout.data_ptr(), dF.data_ptr(), part.data_ptr(), None)
+    torch.cuda.synchronize()
+    got_f = lout.double().cpu().numpy().reshape(gf.shape)
+    assert np.isfinite(got_f).all(), "every adjoint slot must be written exactly once"
+    rel = lambda a, b: np.abs(a - b).max() / np.abs(b).max()  # noqa: E731
+    e_f = rel(got_f, gf.numpy())
+    e_F = rel(dF.double().cpu().numpy().reshape(gF.shape), gF.numpy())
+    a, b = tau_rate_derivatives(p, case.tau)
+    sums = part.view(-1, 2).sum(0).cpu().numpy()
+    e_t = abs(sums[0] * a + sums[1] * b - float(gt)) / abs(float(gt))
+    print(f"{name} {storage}: rel err lam {e_f:.2e}  dF {e_F:.2e}  dtau {e_t:.2e}")
+    assert e_f < 2e-6 and e_F < 2e-5 and e_t < 2e-5
+
+
+def _channel_setup(nx, ny, seed=4):
+    rng = np.random.default_rng(seed)
+    ff = np.zeros((2, ny, nx))
+    ff[0] = 1e-6
+    ff += 1e-7 * rng.standard_normal((2, ny, nx))
+    ut = smooth_random_field((nx, ny), 4, 0.01, seed + 1)
+    return ff.astype(np.float32).astype(np.float64), ut
+
+
+@pytest.mark.gpu
+def test_channel_gradients_vs_float64_autograd_and_fd():
+    """D2Q9 BGK channel 128x64, tau = 0.8, T = 200, per-cell force: dL/dF and dL/dtau."""
+    from pylbm_amd.adjoint import RolloutConfig, rollout, velocity_loss
+
+    nx, ny, T = 128, 64, 200
+    ff, ut = _channel_setup(nx, ny)
+    kw = dict(shape=(nx, ny), tau=0.8, periodic=(True, False), collision="bgk")
+    p = Problem(**kw, force_field=ff)
+    cfg = RolloutConfig(p, T, checkpoint=50)
+    F = _dev(ff).float().requires_grad_(True)
+    tau = torch.tensor(0.8, dtype=torch.float64, device="cuda", requires_grad=True)
+    L = velocity_loss(rollout(cfg, F, tau), _dev(ut))
+    L.backward()
+    gF_hip = F.grad.double().cpu().numpy()
+    gt_hip = float(tau.grad)
+    # float64 autograd of the restated step
+    tl = TorchLBM(R.Case(**kw, force_field=ff))
+    Ft = torch.tensor(ff, requires_grad=True)
+    tt = torch.tensor(0.8, dtype=torch.float64, requires_grad=True)
+    f0 = tl.equilibrium(torch.ones(ny, nx, dtype=torch.float64),
+                        torch.zeros(2, ny, nx, dtype=torch.float64))
+    fT = tl.rollout(f0, Ft, tt, T, segment=50)
+    Lt = 0.5 * ((tl.velocity(fT, Ft) - torch.tensor(ut)) ** 2).sum()
+    gF, gt = torch.autograd.grad(Lt, (Ft, tt))
+    eF = np.abs(gF_hip - gF.numpy()).max() / np.abs(gF.numpy()).max()
+    et = abs(gt_hip - float(gt)) / abs(float(gt))
+    # central finite differences (h = 1e-4 relative) of the float64 C oracle
+    def loss64(tau_, ffield):
+        c = R.Case(**{**kw, "tau": tau_}, force_field=ffield)
+        f = RC.run(c, T)
+        _, u = R.macroscopic(f, c)
+        return 0.5 * float(((u - ut) ** 2).sum())
+
+    h = 1e-4 * 0.8
+    fd_t = (loss64(0.8 + h, ff) - loss64(0.8 - h, ff)) / (2 * h)
+    V = np.random.default_rng(9).standard_normal(ff.shape)
+    hF = 1e-4 * np.abs(ff).max()
+    fd_F = (loss64(0.8, ff + hF * V) - loss64(0.8, ff - hF * V)) / (2 * hF)
+    dd = float((gF_hip * V).sum())
+    print(f"channel: L={float(L):.6e} (f64 {float(Lt):.6e}); dF rel err vs autograd {eF:.2e}; "
+          f"dtau {gt_hip:.6e} vs autograd {float(gt):.6e} ({et:.2e}) vs FD {fd_t:.6e}; "
+          f"<dF,V> {dd:.6e} vs FD {fd_F:.6e}")
+    assert abs(float(L) - float(Lt)) < 1e-5 * float(Lt)
+    assert eF < 1e-4 and et < 1e-4
+    assert abs(gt_hip - fd_t) < 1e-4 * abs(fd_t)
+    assert abs(dd - fd_F) < 1e-4 * abs(fd_F)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("storage,bound", [("f32s", 1e-4), ("f32", 1e-3), ("f16", 5e-2)])
+def test_d3q19_initial_velocity_gradient(storage, bound):
+    """D3Q19 BGK periodic 32^3, tau = 0.7: dL/du0 vs float64 autograd (T = 100)."""
+    from pylbm_amd.adjoint import RolloutConfig, rollout, velocity_loss
+
+    n, T = 32, 100
+    u0 = divergence_free_field(n, 8, 0.02, 1)
+    uo = divergence_free_field(n, 8, 0.02, 2)
+    kw = dict(shape=(n, n, n), tau=0.7, periodic=(True, True, True))
+    cfg = RolloutConfig(Problem(**kw), T, checkpoint=25, storage=storage)
+    U0 = _dev(u0).requires_grad_(True)
+    L = velocity_loss(rollout(cfg, u0=U0), _dev(uo))
+    L.backward()
+    g_hip = U0.grad.cpu().numpy()
+    tl = TorchLBM(R.Case(**kw))
+    Ut = torch.tensor(u0, requires_grad=True)
+    zero = torch.zeros(3, n, n, n, dtype=torch.float64)
+    fT = tl.rollout(tl.equilibrium(torch.ones(n, n, n, dtype=torch.float64), Ut), zero,
+                    torch.tensor(0.7, dtype=torch.float64), T, segment=25)
+    Lt = 0.5 * ((tl.velocity(fT, zero) - torch.tensor(uo)) ** 2).sum()
+    (g,) = torch.autograd.grad(Lt, (Ut,))
+    err = np.abs(g_hip - g.numpy()).max() / np.abs(g.numpy()).max()
+    print(f"D3Q19 32^3 T={T} {storage}: L {float(L):.6e} vs f64 {float(Lt):.6e}; "
+          f"du0 rel err {err:.2e}")
+    assert err < bound

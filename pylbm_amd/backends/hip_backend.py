@@ -31,7 +31,8 @@ except Exception:  # pragma: no cover
     torch = None  # type: ignore[assignment]
 
 _TORCH_DTYPES = {"f32": "float32", "f32s": "float32", "f16": "float16", "bf16": "bfloat16"}
-#: the kernels index a plane with 32-bit offsets (one row segment of slack)
+#: the kernels index a plane with 32-bit node indices (one row segment of slack) and 32-bit
+#: byte offsets, so a plane holds at most 4 GiB
 MAX_NODES = 2**31 - 1 - 256
 
 
@@ -78,8 +79,9 @@ class HipFused(Solver):
             return False, f"{problem.lattice.name} not implemented"
         if problem.collision not in cls.collisions:
             return False, f"{problem.collision} not implemented"
-        if problem.n_nodes > MAX_NODES:
-            return False, "more nodes than the 32-bit in-plane index allows"
+        item = 2 if cls.storage in ("f16", "bf16") else 4
+        if problem.n_nodes > MAX_NODES or problem.n_nodes * item >= 2**32:
+            return False, "a population plane exceeds the kernels' 32-bit offsets"
         return True, ""
 
     def __init__(self, problem: Problem, init: str = "auto") -> None:

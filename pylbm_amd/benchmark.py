@@ -97,3 +97,121 @@ def time_steps(solver: Any, steps: int, warmup: int, repeats: int = 1) -> dict:
 
 def mlups(n_nodes: int, steps: int, seconds: float) -> float:
     return n_nodes * steps / seconds / 1e6
+
+
+# ----------------------------------------------------------------------------- plans
+# The reference's benchmark engine (pylbm/benchmark.py:67-90, 154-236, 242-324): a
+# lid-driven cavity (Re = 1000, U = 0.05, all walls) per size, warm-up, >= 5 repeats of
+# >= min_time seconds, median / min / max MLUPS, effective bandwidth from the 2 q s model.
+PLANS = {
+    "quick": dict(sizes_2d=[1024], sizes_3d=[64], repeats=3, min_time=0.15, copy_mib=1024),
+    "dev": dict(sizes_2d=[256, 1024, 4096], sizes_3d=[64, 128, 256], repeats=5, min_time=0.5,
+                copy_mib=4096),
+    "full": dict(sizes_2d=[256, 512, 1024, 2048, 4096], sizes_3d=[64, 128, 256, 512],
+                 repeats=5, min_time=0.5, copy_mib=4096),
+}
+VARIANTS = ["fused", "fused-aa", "fused-fp16", "fused-bf16", "fused-aa-fp16", "fused-aa-bf16"]
+
+
+def bench_problem(dim: int, n: int, collision: str = "bgk"):
+    from .problem import Problem
+
+    return Problem.cavity(n, re=1000.0, u_lid=0.05, collision=collision, dim=dim)
+
+
+def time_solver(solver: Any, min_time: float, repeats: int) -> tuple[list[float], int]:
+    """Per-repeat wall times and steps per repeat (pylbm/benchmark.py:159-180)."""
+    m = solver.step_multiple
+    steps = 2 * m
+    solver.step(steps)
+    solver.synchronize()
+    while True:
+        t0 = time.perf_counter()
+        solver.step(steps)
+        solver.synchronize()
+        dt = time.perf_counter() - t0
+        if dt >= min_time:
+            break
+        grow = max(2.0, min(10.0, 1.2 * min_time / max(dt, 1e-6)))
+        steps = int(-(-steps * grow // m) * m)
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        solver.step(steps)
+        solver.synchronize()
+        times.append(time.perf_counter() - t0)
+    return times, steps
+
+
+def measure(variant: str, dim: int, n: int, plan: dict, copy_bw: float,
+            collision: str = "bgk", log: Any = print) -> dict | None:
+    """One (variant, size) record in the reference's schema (pylbm/benchmark.py:183-236)."""
+    from . import backends as B
+
+    torch = _torch()
+    cls = B.get("hip", variant)
+    ok, why = cls.available()
+    if not ok:
+        log(f"  skip hip/{variant}: {why}")
+        return None
+    p = bench_problem(dim, n, collision)
+    ok, why = cls.supports(p)
+    if not ok:
+        log(f"  skip hip/{variant} {n}^{dim}: {why}")
+        return None
+    s = cls(p)
+    times, steps = time_solver(s, plan["min_time"], plan["repeats"])
+    ml = [p.n_nodes * steps / t / 1e6 for t in times]
+    med = statistics.median(ml)
+    eff = med * 1e6 * s.bytes_per_node / 1e9
+    rec = {"backend": "hip", "variant": variant, "lattice": p.lattice.name,
+           "collision": collision, "size": list(p.shape), "dtype": "float32",
+           "storage": s.storage, "metric": "MLUPS", "value": med, "repeats": len(times),
+           "median": med, "min": min(ml), "max": max(ml), "steps_per_repeat": steps,
+           "bytes_per_unit": s.bytes_per_node, "effective_GBs": eff,
+           "pct_copy_bw": 100.0 * eff / copy_bw, "pct_spec_bw": 100.0 * eff / SPEC_BANDWIDTH_GBS,
+           "memory_MiB": s.memory_bytes() / 2**20, "notes": cls.description}
+    del s
+    torch.cuda.empty_cache()
+    log(f"  hip {variant:14s} {collision} {'x'.join(map(str, p.shape)):>14s} {med:10.1f} MLUPS "
+        f"{eff:7.1f} GB/s {rec['pct_copy_bw']:5.1f}% copy")
+    return rec
+
+
+def run(plan_name: str = "quick", out: str | None = None, log: Any = print) -> dict:
+    """Run a benchmark plan over every hip variant; write JSON (pylbm/benchmark.py:242-324)."""
+    import json
+    import platform
+
+    torch = _torch()
+    plan = PLANS[plan_name]
+    t0 = time.perf_counter()
+    ceil = copy_bandwidth(plan["copy_mib"])
+    cbw = ceil["copy_bandwidth_GBs"]
+    log(f"[pylbm_amd bench:{plan_name}] measured copy bandwidth {cbw:.1f} GB/s")
+    runs = []
+    for dim, sizes in ((2, plan["sizes_2d"]), (3, plan["sizes_3d"])):
+        for v in VARIANTS:
+            for n in sizes:
+                try:
+                    r = measure(v, dim, n, plan, cbw, log=log)
+                except Exception as e:  # noqa: BLE001 - one failure never ends the run
+                    log(f"  FAIL hip/{v} {n}^{dim}: {type(e).__name__}: {e}")
+                    continue
+                if r:
+                    runs.append(r)
+    for coll in ("trt", "mrt"):
+        n = plan["sizes_2d"][-1]
+        r = measure("fused", 2, n, plan, cbw, collision=coll, log=log)
+        if r:
+            runs.append(r)
+    result = {"machine": {"python": platform.python_version(), "torch": torch.__version__,
+                          "gpu": torch.cuda.get_device_name(0)},
+              "ceiling": {"copy_bandwidth_GBs": cbw, "spec_bandwidth_GBs": SPEC_BANDWIDTH_GBS,
+                          **ceil},
+              "plan": plan_name, "wall_time_s": time.perf_counter() - t0, "runs": runs}
+    if out:
+        with open(out, "w") as fh:
+            json.dump(result, fh, indent=1)
+        log(f"[pylbm_amd bench] wrote {out} ({len(runs)} runs)")
+    return result

@@ -1,0 +1,106 @@
+"""Command line interface: ``python -m pylbm_amd run | validate | bench | backends``.
+
+Mirrors the reference's ``pylbm`` console script (pylbm/cli.py:98-144) for the hip backend:
+``run cavity --backend hip --variant fused|fused-aa|fused-fp16|fused-bf16`` is the
+north-star entry.  (The reference's ``kernels`` subcommand prints DSL source; there is none.)
+"""
+
+from __future__ import annotations
+
+import argparse
+import sys
+import time
+
+
+def _cmd_backends(_: argparse.Namespace) -> None:
+    from . import backends as B
+
+    for b, v, ok, why in B.status():
+        desc = B.get(b, v).description if ok else why
+        print(f"{'ok ' if ok else '-- '} {b + '/' + v:28s} {desc}")
+
+
+def _cmd_run(a: argparse.Namespace) -> None:
+    from .problem import Problem
+    from .validation import make_solver
+
+    if a.case == "cavity":
+        p = Problem.cavity(a.n, re=a.re, u_lid=a.u_lid, collision=a.collision)
+    elif a.case == "cavity3d":
+        p = Problem.cavity(a.n, re=a.re, u_lid=a.u_lid, collision=a.collision, dim=3)
+    else:
+        raise SystemExit(f"unknown case {a.case}")
+    s = make_solver(p, a.backend, a.variant)
+    steps = a.steps - a.steps % s.step_multiple
+    print(f"{s.name}: {p.lattice.name} {'x'.join(map(str, p.shape))}, tau={p.tau:.4f}, "
+          f"{steps} steps")
+    s.step(s.step_multiple)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.synchronize()
+    dt = time.perf_counter() - t0
+    print(f"{p.n_nodes * steps / dt / 1e6:.1f} MLUPS ({dt:.2f} s, not a benchmark: use bench)")
+    if a.out:
+        import numpy as np
+
+        rho, u = s.macroscopic()
+        np.savez_compressed(a.out, rho=rho, u=u)
+        print(f"wrote {a.out}")
+
+
+def _cmd_validate(a: argparse.Namespace) -> None:
+    from .validation import ghia, poiseuille, shear_wave, taylor_green
+
+    v = a.variant
+    r = shear_wave.run_shear_wave(0.8, n=32 if a.quick else 128, backend="hip", variant=v)
+    print(f"shear wave tau=0.8: nu rel err {r['rel_error']:.2e}")
+    r = taylor_green.run_taylor_green(n=16 if a.quick else 48, plane="xz", backend="hip",
+                                      variant=v)
+    print(f"TGV D3Q19: nu rel err {r['rel_error']:.2e}")
+    r = poiseuille.run_poiseuille(16 if a.quick else 32, collision="trt", backend="hip",
+                                  variant=v)
+    print(f"Poiseuille TRT: L2 {r['L2']:.2e}")
+    c = ghia.run_cavity(100, 64 if a.quick else 128, backend="hip", variant=v)
+    print(f"Ghia Re=100: converged={c.converged} Linf={c.errors['Linf']:.4f}")
+
+
+def _cmd_bench(a: argparse.Namespace) -> None:
+    from .benchmark import run
+
+    run("full" if a.full else "dev" if a.dev else "quick", out=a.out)
+
+
+def main(argv: list[str] | None = None) -> None:
+    ap = argparse.ArgumentParser(prog="pylbm_amd", description="gfx950 fused LBM (hip backend)")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    sp = sub.add_parser("backends", help="list backends and whether they run here")
+    sp.set_defaults(func=_cmd_backends)
+    sp = sub.add_parser("run", help="run a case and report throughput")
+    sp.add_argument("case", choices=["cavity", "cavity3d"])
+    sp.add_argument("--n", type=int, default=256)
+    sp.add_argument("--re", type=float, default=100.0)
+    sp.add_argument("--u-lid", type=float, default=0.1)
+    sp.add_argument("--collision", default="bgk", choices=["bgk", "trt", "mrt"])
+    sp.add_argument("--backend", default="hip")
+    sp.add_argument("--variant", default="fused")
+    sp.add_argument("--steps", type=int, default=10000)
+    sp.add_argument("--out", default=None, help="write rho, u (npz)")
+    sp.set_defaults(func=_cmd_run)
+    sp = sub.add_parser("validate", help="analytic checks on the hip backend")
+    sp.add_argument("--quick", action="store_true")
+    sp.add_argument("--variant", default="fused")
+    sp.set_defaults(func=_cmd_validate)
+    sp = sub.add_parser("bench", help="benchmark every hip variant (cavity, Re=1000)")
+    g = sp.add_mutually_exclusive_group()
+    g.add_argument("--quick", action="store_true")
+    g.add_argument("--dev", action="store_true")
+    g.add_argument("--full", action="store_true")
+    sp.add_argument("--out", default=None)
+    sp.set_defaults(func=_cmd_bench)
+    a = ap.parse_args(argv)
+    a.func(a)
+
+
+if __name__ == "__main__":  # pragma: no cover
+    main(sys.argv[1:])

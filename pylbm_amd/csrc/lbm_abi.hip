@@ -13,11 +13,11 @@
 
 namespace plbm {
 template <class L>
-int launch_step(const KArgs& a, int coll, int storage, int pattern, bool field,
+int launch_step(const KArgs& a, int coll, int storage, int pattern, int fmode,
                 const void* src, void* dst, hipStream_t s);
-extern template int launch_step<D2Q9>(const KArgs&, int, int, int, bool, const void*, void*,
+extern template int launch_step<D2Q9>(const KArgs&, int, int, int, int, const void*, void*,
                                       hipStream_t);
-extern template int launch_step<D3Q19>(const KArgs&, int, int, int, bool, const void*, void*,
+extern template int launch_step<D3Q19>(const KArgs&, int, int, int, int, const void*, void*,
                                        hipStream_t);
 }  // namespace plbm
 
@@ -68,26 +68,16 @@ int check_params(const pylbm_params* p, const char* what) {
   if (N >= ((int64_t)1 << 31) - kBlock)
     return fail(PYLBM_ERANGE, "%s: %lld nodes exceed the 32-bit in-plane index", what,
                 (long long)N);
+  const int64_t item = (p->storage == PYLBM_F16 || p->storage == PYLBM_BF16) ? 2 : 4;
+  if (N * item >= ((int64_t)1 << 32))
+    return fail(PYLBM_ERANGE, "%s: a %lld-node plane exceeds the 4 GiB byte offsets of the "
+                "kernels", what, (long long)N);
   const int last = p->lattice == PYLBM_D2Q9 ? 1 : 2;
   if (p->ulid != 0.0f && p->periodic[last])
     return fail(PYLBM_EINVAL, "%s: a moving lid needs a wall on the last axis", what);
   if (p->lattice == PYLBM_D2Q9 && p->force_field == nullptr && p->force[2] != 0.0f)
     return fail(PYLBM_EINVAL, "%s: D2Q9 has no z force", what);
   return 0;
-}
-
-// Nodes per lane of the fused step: the default per storage type, or PYLBM_HIP_VEC
-// (1 = one node per lane; fp32 2 or 4; 16-bit 4 or 8) to compare the kernels.
-int vec_choice(int storage, int nx) {
-  static const int forced = [] {
-    const char* e = getenv("PYLBM_HIP_VEC");
-    return e ? atoi(e) : 0;
-  }();
-  const bool wide = storage == PYLBM_F32 || storage == PYLBM_F32S;
-  const int lo = wide ? 2 : 4, hi = wide ? 4 : 8;
-  int v = 4;
-  if (forced == 1 || forced == lo || forced == hi) v = forced;
-  return (v > 1 && nx % v == 0) ? v : 1;
 }
 
 int make_kargs(const pylbm_params* p, KArgs& a, const char* what) {
@@ -104,8 +94,8 @@ int make_kargs(const pylbm_params* p, KArgs& a, const char* what) {
   for (int k = 0; k < dims; ++k)
     if (p->periodic[k]) a.per |= 1 << k;
   a.remap = (a.nblk % 8 == 0) ? 1 : 0;
-  // vectorised kernel whenever rows split into whole vectors
-  a.vec = vec_choice(p->storage, p->nx);
+  a.div_nseg = make_fastdiv((uint32_t)a.nseg);
+  a.div_ny = make_fastdiv((uint32_t)p->ny);
   a.ulid = p->ulid;
   a.fx = p->force[0];
   a.fy = p->force[1];
@@ -120,10 +110,12 @@ int make_kargs(const pylbm_params* p, KArgs& a, const char* what) {
 
 int step_impl(const pylbm_params* p, const KArgs& a, const void* src, void* dst, int pattern,
               hipStream_t s) {
-  const bool field = p->force_field != nullptr;
+  const int fmode = p->force_field != nullptr ? kForceField
+                    : (a.fx != 0.0f || a.fy != 0.0f || a.fz != 0.0f) ? kForceUniform
+                                                                     : kForceNone;
   if (p->lattice == PYLBM_D2Q9)
-    return launch_step<D2Q9>(a, p->collision, p->storage, pattern, field, src, dst, s);
-  return launch_step<D3Q19>(a, p->collision, p->storage, pattern, field, src, dst, s);
+    return launch_step<D2Q9>(a, p->collision, p->storage, pattern, fmode, src, dst, s);
+  return launch_step<D3Q19>(a, p->collision, p->storage, pattern, fmode, src, dst, s);
 }
 
 AuxArgs aux_from(const pylbm_params* p) {

@@ -103,11 +103,25 @@ template <> struct Codec<PYLBM_BF16> {
   }
 };
 
-__device__ __forceinline__ float Codec_ld(float v) { return v; }
-__device__ __forceinline__ float Codec_ld(_Float16 v) { return (float)v; }
-__device__ __forceinline__ float Codec_ld(uint16_t v) { return __uint_as_float(((uint32_t)v) << 16); }
-
 // ------------------------------------------------------------------------ kernel args
+// Division by a run-time constant d on the scalar unit: floor(n / d) = (n * m) >> k with
+// k = 32 + ceil(log2 d), m = ceil(2^k / d), exact for every n < 2^31 (the error term
+// n (m - 2^k/d) / 2^k stays below 1/(2d)).  GPUs have no integer divide; without this,
+// the row / column split of the block index would be computed on the vector unit.
+struct FastDiv {
+  uint64_t m;
+  uint32_t k;
+  __host__ __device__ uint32_t div(uint32_t n) const { return (uint32_t)(((uint64_t)n * m) >> k); }
+};
+
+inline FastDiv make_fastdiv(uint32_t d) {
+  uint32_t l = 0;
+  while (((uint64_t)1 << l) < d) ++l;
+  const uint32_t k = 32 + l;
+  const unsigned __int128 p = (unsigned __int128)1 << k;
+  return FastDiv{(uint64_t)((p + d - 1) / d), k};
+}
+
 struct KArgs {
   int nx, ny, nz;
   int N;       // nodes per plane (< 2^31)
@@ -115,7 +129,7 @@ struct KArgs {
   int nblk;    // total workgroups
   int per;     // bit a: axis a periodic
   int remap;   // 1: XCD-aware workgroup -> row-segment mapping
-  int vec;     // >1: use the vectorised kernel (nodes per lane for its storage type)
+  FastDiv div_nseg, div_ny;
   float ulid;
   float fx, fy, fz;
   float r0, r1, r2, r3;
@@ -137,12 +151,13 @@ __device__ __forceinline__ int remap_block(int b, int nblk, int remap) {
 // feq_i - w_i = w_i (drho + rho (3 cu + 4.5 cu^2 - 1.5 u^2)), every operator below is the
 // reference's (pylbm/core.py:203-247) with f and feq replaced by their deviations (the
 // weights cancel in f - feq and sum_i c_i w_i = 0), so the small non-equilibrium part is
-// never formed as a difference of two O(w) numbers.  F: Guo force at the node (zero when
-// the problem has none, which leaves every term exact).
-template <class L, int COLL>
+// never formed as a difference of two O(w) numbers.  F: Guo force at the node; FORCE =
+// false drops every Guo term at compile time (the no-force kernels).  1/rho uses the
+// hardware reciprocal (v_rcp_f32, 1 ulp).
+template <class L, int COLL, bool FORCE>
 __device__ __forceinline__ void collide_node(const float (&h)[L::Q], float (&d)[L::Q],
-                                             const float F[3], bool force, float r0, float r1,
-                                             float r2, float r3) {
+                                             const float F[3], float r0, float r1, float r2,
+                                             float r3) {
   constexpr int Q = L::Q;
   float drho = 0.f, j0 = 0.f, j1 = 0.f, j2 = 0.f;
 #pragma unroll
@@ -153,12 +168,12 @@ __device__ __forceinline__ void collide_node(const float (&h)[L::Q], float (&d)[
     if (L::D == 3 && L::c(i, 2)) j2 += L::c(i, 2) * h[i];
   }
   const float rho = 1.0f + drho;
-  const float inv = 1.0f / rho;
-  const float ux = (j0 + 0.5f * F[0]) * inv;
-  const float uy = (j1 + 0.5f * F[1]) * inv;
-  const float uz = (L::D == 3) ? (j2 + 0.5f * F[2]) * inv : 0.0f;
+  const float inv = __builtin_amdgcn_rcpf(rho);
+  const float ux = (FORCE ? j0 + 0.5f * F[0] : j0) * inv;
+  const float uy = (FORCE ? j1 + 0.5f * F[1] : j1) * inv;
+  const float uz = (L::D == 3) ? (FORCE ? j2 + 0.5f * F[2] : j2) * inv : 0.0f;
   const float usq = ux * ux + uy * uy + uz * uz;
-  const float uf = ux * F[0] + uy * F[1] + uz * F[2];
+  const float uf = FORCE ? ux * F[0] + uy * F[1] + uz * F[2] : 0.0f;
   float neq[Q], S[Q];  // neq = f - feq = h - (feq - w)
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
@@ -167,12 +182,12 @@ __device__ __forceinline__ void collide_node(const float (&h)[L::Q], float (&d)[
     const float poly = cu * (3.0f + 4.5f * cu) - 1.5f * usq;
     neq[i] = h[i] - w * (drho + rho * poly);
     const float cf = L::c(i, 0) * F[0] + L::c(i, 1) * F[1] + L::c(i, 2) * F[2];
-    S[i] = force ? w * (3.0f * (cf - uf) + 9.0f * cu * cf) : 0.0f;
+    S[i] = FORCE ? w * (3.0f * (cf - uf) + 9.0f * cu * cf) : 0.0f;
   }
   if constexpr (COLL == PYLBM_BGK) {
     const float g = 1.0f - 0.5f * r0;
 #pragma unroll
-    for (int i = 0; i < Q; ++i) d[i] = h[i] - r0 * neq[i] + g * S[i];
+    for (int i = 0; i < Q; ++i) d[i] = FORCE ? h[i] - r0 * neq[i] + g * S[i] : h[i] - r0 * neq[i];
   } else if constexpr (COLL == PYLBM_TRT) {
     const float gp = 1.0f - 0.5f * r0, gm = 1.0f - 0.5f * r1;
 #pragma unroll
@@ -181,7 +196,7 @@ __device__ __forceinline__ void collide_node(const float (&h)[L::Q], float (&d)[
       const float np_ = 0.5f * (neq[i] + neq[o]);
       const float nm_ = 0.5f * (neq[i] - neq[o]);
       const float sp_ = 0.5f * (S[i] + S[o]), sm_ = 0.5f * (S[i] - S[o]);
-      d[i] = h[i] - r0 * np_ - r1 * nm_ + gp * sp_ + gm * sm_;
+      d[i] = FORCE ? h[i] - r0 * np_ - r1 * nm_ + gp * sp_ + gm * sm_ : h[i] - r0 * np_ - r1 * nm_;
     }
   } else {
     static_assert(COLL != PYLBM_MRT || L::Q == 9, "MRT is D2Q9 only");
@@ -194,9 +209,11 @@ __device__ __forceinline__ void collide_node(const float (&h)[L::Q], float (&d)[
       for (int i = 0; i < 9; ++i) {
         if (mrt_m(k, i) == 0) continue;
         mne += (float)mrt_m(k, i) * neq[i];
-        ms += (float)mrt_m(k, i) * S[i];
+        if (FORCE) ms += (float)mrt_m(k, i) * S[i];
       }
-      dm[k] = (-s[k] * mne + (1.0f - 0.5f * s[k]) * ms) / (float)mrt_norm(k);
+      // conserved moments (s = 0) change only through the force
+      const float rn = (float)(1.0 / mrt_norm(k));
+      dm[k] = FORCE ? (-s[k] * mne + (1.0f - 0.5f * s[k]) * ms) * rn : -s[k] * mne * rn;
     }
 #pragma unroll
     for (int i = 0; i < 9; ++i) {

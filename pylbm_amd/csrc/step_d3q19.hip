@@ -3,6 +3,6 @@
 #include "lbm_step.hpp"
 
 namespace plbm {
-template int launch_step<D3Q19>(const KArgs&, int, int, int, bool, const void*, void*,
+template int launch_step<D3Q19>(const KArgs&, int, int, int, int, const void*, void*,
                                 hipStream_t);
 }  // namespace plbm

@@ -90,6 +90,76 @@ def run_variant(variant: str, problem, init, steps: int, warmup: int, repeats: i
     return rec
 
 
+def adjoint_leg(name: str, repeats: int = 3) -> dict:
+    """Forward+adjoint throughput of a checkpointed differentiable rollout vs forward only.
+
+    channel: D2Q9 BGK 1024x256 (periodic x, walls in y), tau 0.8, per-cell Guo force
+             (1e-6 in +x plus 1e-7 N(0,1) noise), T = 2000, checkpoints every 100 steps,
+             loss 0.5 sum |u - u_target|^2 with a smooth 4-mode target, gradients w.r.t.
+             the force field and tau; states stored as shifted fp32.
+    box3d:   D3Q19 BGK periodic 128^3, tau 0.7, divergence-free initial velocity (8 modes,
+             amplitude 0.02), T = 500, checkpoints every 50, fp16-shifted forward storage,
+             fp32 adjoint, gradient w.r.t. the initial velocity.
+    MLUPS(fwd+adj) = nodes * T / (forward-with-checkpoints + backward time).
+    """
+    import torch
+
+    from pylbm_amd import backends as B
+    from pylbm_amd.adjoint import RolloutConfig, rollout, velocity_loss
+    from pylbm_amd.fields import divergence_free_field, smooth_random_field
+    from pylbm_amd.problem import Problem
+
+    rng = np.random.default_rng(SEED)
+    if name == "channel":
+        nx, ny, T, C, storage, variant = 1024, 256, 2000, 100, "f32s", "fused"
+        ff = np.zeros((2, ny, nx))
+        ff[0] = 1e-6
+        ff += 1e-7 * rng.standard_normal((2, ny, nx))
+        p = Problem((nx, ny), tau=0.8, periodic=(True, False), force_field=ff)
+        target = torch.as_tensor(smooth_random_field((nx, ny), 4, 0.01, SEED + 1), device="cuda")
+        F = torch.as_tensor(ff, dtype=torch.float32, device="cuda")
+        u0 = None
+    else:
+        n, T, C, storage, variant = 128, 500, 50, "f16", "fused-fp16"
+        p = Problem((n, n, n), tau=0.7, periodic=(True, True, True))
+        target = torch.as_tensor(divergence_free_field(n, 8, 0.02, SEED + 2), device="cuda")
+        u0 = torch.as_tensor(divergence_free_field(n, 8, 0.02, SEED + 1), device="cuda")
+        F = None
+    N = p.n_nodes
+    # forward only: the same kernels, T steps, no checkpoints
+    s = B.make("hip", variant, p, init="none")
+    s.init_equilibrium(None, u0)
+    s.step(T // 10)
+    s.synchronize()
+    fwd = []
+    for _ in range(repeats):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        s.step(T)
+        e1.record()
+        e1.synchronize()
+        fwd.append(e0.elapsed_time(e1) / 1e3)
+    del s
+    cfg = RolloutConfig(p, T, checkpoint=C, storage=storage)
+    times, L = [], None
+    for r in range(repeats + 1):
+        Fr = F.clone().requires_grad_(True) if F is not None else None
+        tau = torch.tensor(p.tau, dtype=torch.float64, device="cuda", requires_grad=F is not None)
+        ur = u0.clone().requires_grad_(True) if u0 is not None else None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        L = velocity_loss(rollout(cfg, Fr, tau, ur), target)
+        L.backward()
+        torch.cuda.synchronize()
+        if r:  # the first pass warms up allocations
+            times.append(time.perf_counter() - t0)
+    torch.cuda.empty_cache()
+    t_fwd, t_fa = statistics.median(fwd), statistics.median(times)
+    return {"case": name, "nodes": N, "steps": T, "checkpoint": C, "storage": storage,
+            "forward_MLUPS": N * T / t_fwd / 1e6, "fwd_adjoint_MLUPS": N * T / t_fa / 1e6,
+            "ratio": t_fwd / t_fa, "loss": float(L.detach())}
+
+
 def cpu_baseline(n: int, steps: int) -> dict:
     """The float64 NumPy roll oracle on the host, D2Q9 periodic shear wave n^2."""
     from oracle import lbm_ref as R
@@ -126,6 +196,9 @@ def main() -> int:
     ap.add_argument("--profile", action="store_true",
                     help="only the headline kernel and the copy kernel (for rocprofv3)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint legs")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the forward+adjoint legs (prints them, no JSON line)")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--n2", type=int, default=4096)
     ap.add_argument("--n3", type=int, default=512)
@@ -146,6 +219,12 @@ def main() -> int:
     _lib.load()
     dev = torch.cuda.get_device_name(0)
     log(f"device {dev}")
+    if a.adjoint_only:
+        for case in ("channel", "box3d"):
+            r = adjoint_leg(case)
+            log(f"fwd+adjoint {case}: {r['fwd_adjoint_MLUPS']:.0f} MLUPS vs forward "
+                f"{r['forward_MLUPS']:.0f} ({100 * r['ratio']:.1f}%)")
+        return 0
     n = a.n2
     rho, u = shear_wave_init(n, SEED)
 
@@ -202,6 +281,14 @@ def main() -> int:
         r["pct_copy"] = 100.0 * r["GBs"] / cbw
         r["pct_spec"] = 100.0 * r["GBs"] / SPEC_BANDWIDTH_GBS
 
+    adjoint = []
+    if not a.quick and not a.no_adjoint:
+        for case in ("channel", "box3d"):
+            r = adjoint_leg(case)
+            adjoint.append(r)
+            log(f"  fwd+adjoint {case}: {r['fwd_adjoint_MLUPS']:.0f} MLUPS vs forward "
+                f"{r['forward_MLUPS']:.0f} ({100 * r['ratio']:.1f}%)")
+
     t_launch = head["us_per_step"] * 1e-6
     achieved = head["bytes_per_update"] * n * n / t_launch / 1e9
     traffic = load_traffic()
@@ -233,6 +320,7 @@ def main() -> int:
         "copy_ceiling": ceil,
         "pct_copy": head["pct_copy"],
         "variants": variants,
+        "fwd_adjoint": adjoint,
         "device": dev,
     }
     print(json.dumps(line), flush=True)

@@ -90,6 +90,16 @@ int pylbm_hip_step(const pylbm_params* p, const void* src, void* dst, int patter
 int pylbm_hip_run(const pylbm_params* p, void* a, void* b, int64_t nsteps, void* stream);
 
 /*
+ * nsteps two-buffer steps from src, writing the state after step k to dst[k] (a host array
+ * of nsteps device pointers; each dst[k] must differ from the buffer it is computed from).
+ * Enqueues the whole loop from C: used by the differentiable rollouts to store checkpoints
+ * and to recompute a checkpoint segment.  No reference counterpart (the reference has no
+ * rollout storage); a generalisation of FusedSolver._advance (pylbm/backends/_fused.py:116-124).
+ */
+int pylbm_hip_run_store(const pylbm_params* p, const void* src, void* const* dst,
+                        int64_t nsteps, void* stream);
+
+/*
  * Canonical float64 state (q, N) -> stored layout (shift by -w_i for 16-bit storage,
  * rounded float64 -> float32 -> storage; AA slot swap).
  * Replaces FusedSolver._encode / set_populations (pylbm/backends/_fused.py:83-93).
@@ -162,6 +172,14 @@ int pylbm_hip_adjoint_seed(const pylbm_params* p, const void* f_T, const float* 
  * for workgroup g; the caller zeroes dtau_part once and sums it at the end. */
 int pylbm_hip_adjoint_step(const pylbm_params* p, const void* f_t, const float* lam_in,
                            float* lam_out, float* dF, double* dtau_part, void* stream);
+
+/* nsteps adjoint steps over a stored segment, in reverse: for k = nsteps-1 .. 0,
+ * lam_out = adjoint of step(states[k]) applied to lam_in, ping-ponging lam_a (input on
+ * entry) and lam_b; *which = 0 if the final adjoint state is in lam_a, 1 if in lam_b.
+ * dF and dtau_part accumulate as in pylbm_hip_adjoint_step. */
+int pylbm_hip_adjoint_run(const pylbm_params* p, const void* const* states, int64_t nsteps,
+                          float* lam_a, float* lam_b, float* dF, double* dtau_part, int* which,
+                          void* stream);
 
 /* Gradient of the initial equilibrium f0 = feq(rho0, u0) (float64 fields; NULL rho0 = 1,
  * NULL u0 = 0): du0 (d, N) and drho0 (N) in float64 from lam0 = dL/df0 (either output may

@@ -31,7 +31,7 @@ EXPORTS = (
     "pylbm_hip_encode", "pylbm_hip_decode", "pylbm_hip_init_equilibrium",
     "pylbm_hip_macroscopic", "pylbm_hip_reduce", "pylbm_hip_copy",
     "pylbm_hip_adjoint_workspace", "pylbm_hip_adjoint_seed", "pylbm_hip_adjoint_step",
-    "pylbm_hip_adjoint_init",
+    "pylbm_hip_adjoint_init", "pylbm_hip_run_store", "pylbm_hip_adjoint_run",
 )
 
 
@@ -93,6 +93,9 @@ def load() -> Any:
         "pylbm_hip_adjoint_seed": ([pp, vp, vp, vp, vp, vp], i32),
         "pylbm_hip_adjoint_step": ([pp, vp, vp, vp, vp, vp, vp], i32),
         "pylbm_hip_adjoint_init": ([pp, vp, vp, vp, vp, vp, vp], i32),
+        "pylbm_hip_run_store": ([pp, vp, ctypes.POINTER(vp), i64, vp], i32),
+        "pylbm_hip_adjoint_run": ([pp, ctypes.POINTER(vp), i64, vp, vp, vp, vp,
+                                   ctypes.POINTER(ctypes.c_int), vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

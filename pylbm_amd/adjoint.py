@@ -122,6 +122,13 @@ class _Engine:
         return u
 
 
+def _ptr_array(tensors: list) -> Any:
+    arr = (ctypes.c_void_p * max(len(tensors), 1))()
+    for k, t in enumerate(tensors):
+        arr[k] = t.data_ptr()
+    return arr
+
+
 class _RolloutFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, force_field, tau, u0, rho0, cfg):  # noqa: ANN001
@@ -135,20 +142,22 @@ class _RolloutFn(torch.autograd.Function):
         last0 = (K - 1) * C  # first step of the last segment
         f0 = eng.state()
         eng.init(r0d, u0d, f0)
-        ckpts = [f0]          # f^{kC}, k = 0 .. K-1
-        last = []             # f^t, t = last0 .. T-1 (inputs of the last segment's steps)
+        # where each f^{t+1} goes: its own buffer if it is a checkpoint, an input of the last
+        # segment or f^T, else one of two scratch buffers (never the buffer it is read from)
         scratch = [eng.state(), eng.state()]
+        ckpts, last, dst = [f0], [], []
         cur = f0
         for t in range(T):
             if t >= last0:
                 last.append(cur)
             keep = (t + 1 == T) or ((t + 1) % C == 0) or (t + 1 >= last0)
             nxt = eng.state() if keep else (scratch[0] if cur is not scratch[0] else scratch[1])
-            eng.step(cur, nxt)
             if t + 1 < T and (t + 1) % C == 0:
                 ckpts.append(nxt)
+            dst.append(nxt)
             cur = nxt
-        del scratch
+        _lib.call("pylbm_hip_run_store", eng.ref, _p(f0), _ptr_array(dst), T, _s())
+        del scratch, dst
         fT = cur
         u = eng.velocity(fT)
         ctx.eng, ctx.ckpts, ctx.last, ctx.fT = eng, ckpts, last, fT
@@ -166,31 +175,32 @@ class _RolloutFn(torch.autograd.Function):
         K = (T + C - 1) // C
         q, d, N = eng.q, eng.d, eng.N
         G = grad_u.detach().to(torch.float32).contiguous()
-        lam = torch.empty(q * N, dtype=torch.float32, device=eng.dev)
-        lam2 = torch.empty_like(lam)
+        lam = [torch.empty(q * N, dtype=torch.float32, device=eng.dev) for _ in range(2)]
         dF = torch.zeros(d * N, dtype=torch.float32, device=eng.dev) if eng.ff is not None \
             else None
         nw = int(_lib.load().pylbm_hip_adjoint_workspace(eng.ref))
         part = torch.zeros(max(nw, 2), dtype=torch.float64, device=eng.dev)
-        _lib.call("pylbm_hip_adjoint_seed", eng.ref, _p(ctx.fT), _p(G), _p(lam), _p(dF), _s())
+        _lib.call("pylbm_hip_adjoint_seed", eng.ref, _p(ctx.fT), _p(G), _p(lam[0]), _p(dF),
+                  _s())
+        cur = 0  # index of the lam buffer holding the current adjoint state
+        which = ctypes.c_int(0)
         seg_buf: list[torch.Tensor] = []
         for k in range(K - 1, -1, -1):
-            t0 = k * C
-            n = min(C, T - t0)
+            n = min(C, T - k * C)
             if k == K - 1:
                 states = ctx.last
-            else:
-                while len(seg_buf) < n:
+            else:  # recompute f^{kC+1} .. f^{kC+n-1} from the checkpoint
+                while len(seg_buf) < n - 1:
                     seg_buf.append(eng.state())
-                states = [ctx.ckpts[k]] + seg_buf[1:n]
-                for j in range(1, n):
-                    eng.step(states[j - 1], states[j])
-            for j in range(n - 1, -1, -1):
-                _lib.call("pylbm_hip_adjoint_step", eng.ref, _p(states[j]), _p(lam), _p(lam2),
-                          _p(dF), _p(part), _s())
-                lam, lam2 = lam2, lam
+                _lib.call("pylbm_hip_run_store", eng.ref, _p(ctx.ckpts[k]),
+                          _ptr_array(seg_buf[:n - 1]), n - 1, _s())
+                states = [ctx.ckpts[k]] + seg_buf[:n - 1]
+            _lib.call("pylbm_hip_adjoint_run", eng.ref, _ptr_array(states), n, _p(lam[cur]),
+                      _p(lam[1 - cur]), _p(dF), _p(part), ctypes.byref(which), _s())
+            cur = cur if which.value == 0 else 1 - cur
             if k == K - 1:
                 ctx.last = None
+        lam0 = lam[cur]
         need_F, need_tau, need_u, need_rho = ctx.needs
         du0 = drho0 = None
         if need_u or need_rho:
@@ -198,7 +208,7 @@ class _RolloutFn(torch.autograd.Function):
                 if need_u else None
             dr = torch.empty(eng.p.array_shape, dtype=torch.float64, device=eng.dev) \
                 if need_rho else None
-            _lib.call("pylbm_hip_adjoint_init", eng.ref, _p(ctx.r0d), _p(ctx.u0d), _p(lam),
+            _lib.call("pylbm_hip_adjoint_init", eng.ref, _p(ctx.r0d), _p(ctx.u0d), _p(lam0),
                       _p(du), _p(dr), _s())
             du0, drho0 = du, dr
         dtau = None
@@ -226,7 +236,7 @@ def rollout(cfg: RolloutConfig, force_field: torch.Tensor | None = None,
         tau = torch.tensor(p.tau, dtype=torch.float64, device="cuda")
     elif not torch.is_tensor(tau):
         tau = torch.tensor(float(tau), dtype=torch.float64, device="cuda")
-    if float(tau) <= 0.5:
+    if float(tau.detach()) <= 0.5:
         raise ValueError("tau must be > 0.5")
     return _RolloutFn.apply(force_field, tau, u0, rho0, cfg)
 

@@ -187,6 +187,26 @@ int pylbm_hip_run(const pylbm_params* p, void* a_buf, void* b_buf, int64_t nstep
   return 0;
 }
 
+int pylbm_hip_run_store(const pylbm_params* p, const void* src, void* const* dst,
+                        int64_t nsteps, void* stream) {
+  if (int e = check_params(p, "pylbm_hip_run_store")) return e;
+  if (p->aa) return fail(PYLBM_EINVAL, "pylbm_hip_run_store: two-buffer layout only");
+  if (nsteps < 0) return fail(PYLBM_EINVAL, "pylbm_hip_run_store: nsteps < 0");
+  if (nsteps > 0 && (!src || !dst)) return fail(PYLBM_EINVAL, "pylbm_hip_run_store: NULL buffer");
+  KArgs a;
+  if (int e = make_kargs(p, a, "pylbm_hip_run_store")) return e;
+  const void* cur = src;
+  for (int64_t k = 0; k < nsteps; ++k) {
+    if (!dst[k] || dst[k] == cur)
+      return fail(PYLBM_EINVAL, "pylbm_hip_run_store: dst[%lld] is NULL or equals its source",
+                  (long long)k);
+    if (int e = step_impl(p, a, cur, dst[k], PYLBM_AB, (hipStream_t)stream))
+      return hip_fail(e, "pylbm_hip_run_store");
+    cur = dst[k];
+  }
+  return 0;
+}
+
 int pylbm_hip_encode(const pylbm_params* p, const double* canon, void* f, void* stream) {
   if (int e = check_params(p, "pylbm_hip_encode")) return e;
   if (!canon || !f) return fail(PYLBM_EINVAL, "pylbm_hip_encode: NULL buffer");
@@ -295,6 +315,34 @@ int pylbm_hip_adjoint_step(const pylbm_params* p, const void* f_t, const float* 
                                       p->force_field != nullptr, f_t, lam_in, lam_out, dF,
                                       dtau_part, (hipStream_t)stream),
                   "pylbm_hip_adjoint_step");
+}
+
+int pylbm_hip_adjoint_run(const pylbm_params* p, const void* const* states, int64_t nsteps,
+                          float* lam_a, float* lam_b, float* dF, double* dtau_part, int* which,
+                          void* stream) {
+  if (int e = check_adjoint(p, "pylbm_hip_adjoint_run")) return e;
+  if (nsteps < 0 || !which) return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_run: bad arguments");
+  if (nsteps > 0 && (!states || !lam_a || !lam_b || lam_a == lam_b || !dtau_part))
+    return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_run: NULL or aliased buffer");
+  if (p->force_field && !dF)
+    return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_run: a force field needs dF");
+  KArgs a;
+  if (int e = make_kargs(p, a, "pylbm_hip_adjoint_run")) return e;
+  float* lin = lam_a;
+  float* lout = lam_b;
+  for (int64_t k = nsteps - 1; k >= 0; --k) {
+    if (!states[k]) return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_run: states[%lld] is NULL",
+                                (long long)k);
+    if (int e = launch_adjoint_step(p->lattice, a, p->collision, p->storage,
+                                    p->force_field != nullptr, states[k], lin, lout, dF,
+                                    dtau_part, (hipStream_t)stream))
+      return hip_fail(e, "pylbm_hip_adjoint_run");
+    float* t = lin;
+    lin = lout;
+    lout = t;
+  }
+  *which = (lin == lam_a) ? 0 : 1;
+  return 0;
 }
 
 int pylbm_hip_adjoint_init(const pylbm_params* p, const double* rho0, const double* u0,

@@ -1,0 +1,164 @@
+// Memory access of the fused kernels (step and adjoint): raw buffer loads / stores on
+// uniform bases, the row-segment decomposition of a workgroup, per-lane x offsets, the
+// y/z part of a link, and the pull gather.
+//
+//   * one lane per node, one workgroup = 256 lanes = one contiguous segment of one x-row
+//     (fixed y, z), so every wave64 access to a plane covers 64 consecutive, naturally
+//     aligned elements;
+//   * the y/z part of every link (row wrap, wall, lid) is decided once per workgroup on the
+//     scalar unit (the block index is split with an exact reciprocal multiply, FastDiv,
+//     since a run-time integer division would run on the vector unit) and folded into one
+//     base address per direction;
+//   * the lane part of every address is one of three 32-bit byte offsets (own node, x-1,
+//     x+1, periodic wrap applied) shared by all directions, and accesses are raw buffer
+//     instructions on a descriptor built from the uniform base, so interior lanes spend no
+//     vector instructions on addressing;
+//   * the gather issues all its loads back to back; lanes on an x wall read their
+//     bounce-back values afterwards, in one divergent block, into separate registers.
+#pragma once
+
+#include "lbm_common.hpp"
+
+namespace plbm {
+
+// Raw buffer access at base + off: base is uniform (a 4-SGPR descriptor, no bounds limit:
+// every offset the kernels form is in range by construction), off a 32-bit lane offset.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc_at(const void* base) {
+  // word 3 = 0x00020000: 32-bit data format, as gfx9-family raw buffers require
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)0xFFFFFFFF,
+                                           0x00020000);
+}
+
+template <class T>
+__device__ __forceinline__ T ld_at(const char* base, uint32_t off) {
+  if constexpr (sizeof(T) == 4)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rsrc_at(base), off, 0, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(rsrc_at(base), off, 0, 0));
+}
+
+// AUX = 2 marks the store non-temporal (nt): written once, not re-read by the same kernel.
+template <int AUX, class T>
+__device__ __forceinline__ void st_at(char* base, uint32_t off, T v) {
+  if constexpr (sizeof(T) == 4)
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), rsrc_at(base), off, 0,
+                                          AUX);
+  else
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, v), rsrc_at(base), off, 0,
+                                          AUX);
+}
+
+// A lane's node and the element indices of its own node (n) and its x-1 / x+1 neighbours
+// in the same row (nm, np), with the periodic wrap applied; on an x wall the neighbour
+// index is the own node (a valid placeholder) and wall_lo / wall_hi is set.  o0 / om / op
+// are the byte offsets of the same elements for element size TS.
+struct Lane {
+  int x, y, z;
+  uint32_t n, nm, np, o0, om, op;
+  bool wall_lo, wall_hi;
+};
+
+template <uint32_t TS>
+__device__ __forceinline__ bool lane_setup(const KArgs& a, int blk, Lane& l) {
+  const int row = (int)a.div_nseg.div((uint32_t)blk);
+  const int seg = blk - row * a.nseg;
+  l.z = (int)a.div_ny.div((uint32_t)row);
+  l.y = row - l.z * a.ny;
+  l.x = seg * kBlock + (int)threadIdx.x;
+  const int nx = a.nx;
+  const bool perx = a.per & 1;
+  l.n = (uint32_t)(l.x + nx * (l.y + a.ny * l.z));
+  int dxm1 = -1, dxp1 = 1;
+  l.wall_lo = l.wall_hi = false;
+  if (l.x == 0) {
+    if (perx) dxm1 = nx - 1;
+    else { dxm1 = 0; l.wall_lo = true; }
+  }
+  if (l.x == nx - 1) {
+    if (perx) dxp1 = -(nx - 1);
+    else { dxp1 = 0; l.wall_hi = true; }
+  }
+  l.nm = (uint32_t)((int)l.n + dxm1);
+  l.np = (uint32_t)((int)l.n + dxp1);
+  l.o0 = l.n * TS;
+  l.om = l.nm * TS;
+  l.op = l.np * TS;
+  return l.x < nx;
+}
+
+// The y/z part of link i seen from row (y, z) towards y + sign c_y, z + sign c_z
+// (sign = -1: the pull source, +1: the push target): wall (the link leaves the box through
+// a y or z wall), top (through the lid: the high wall of the last axis) and the element
+// offset of the wrapped target row relative to the own row.  Uniform over a workgroup.
+struct RowLink {
+  bool wall, top;
+  ptrdiff_t delta;
+};
+
+template <class L>
+__device__ __forceinline__ RowLink row_link(const KArgs& a, int y, int z, int i, int sign) {
+  const int cy = sign * L::c(i, 1), cz = sign * L::c(i, 2);
+  const int ny = a.ny, nz = a.nz;
+  const bool pery = (a.per >> 1) & 1, perz = (a.per >> 2) & 1;
+  int ty = y + cy, tz = z + cz;
+  RowLink r{false, false, 0};
+  if (cy != 0 && (ty < 0 || ty >= ny)) {
+    if (pery) ty += (ty < 0) ? ny : -ny;
+    else { r.wall = true; if (L::D == 2 && ty >= ny) r.top = true; }
+  }
+  if (L::D == 3 && cz != 0 && (tz < 0 || tz >= nz)) {
+    if (perz) tz += (tz < 0) ? nz : -nz;
+    else { r.wall = true; if (tz >= nz) r.top = true; }
+  }
+  r.delta = (ptrdiff_t)a.nx * (ty - y) + (ptrdiff_t)a.nx * ny * (tz - z);
+  return r;
+}
+
+// Pull gather for pattern PAT (pylbm/codegen.py:10-16): h_i = f_i - w_i of the population
+// that streams into the lane's node, with half-way bounce-back and Ladd's lid term.
+//   AB      : read slot i at x - c_i, or own slot opp(i) on a wall link
+//   AA odd  : read slot opp(i) at x - c_i, or own slot i on a wall link
+//   AA even : read own slot i
+template <class L, int ST, int PAT>
+__device__ __forceinline__ void gather(const KArgs& a, const Lane& l, const char* Ab,
+                                       size_t plane, float (&h)[L::Q]) {
+  using C = Codec<ST>;
+  using T = typename C::T;
+  constexpr int Q = L::Q;
+  constexpr uint32_t TS = sizeof(T);
+  T t[Q];
+  float lid[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const int cx = L::c(i, 0);
+    const int own_bb = (PAT == PYLBM_AB) ? L::opp(i) : i;
+    const int far = (PAT == PYLBM_AB) ? i : L::opp(i);
+    lid[i] = 0.0f;
+    if (PAT == PYLBM_AA_EVEN || i == 0) {
+      const int slot = (PAT == PYLBM_AA_ODD) ? L::opp(i) : i;
+      t[i] = ld_at<T>(Ab + slot * plane, l.o0);
+      continue;
+    }
+    const RowLink r = row_link<L>(a, l.y, l.z, i, -1);
+    const char* base = r.wall ? Ab + own_bb * plane : Ab + far * plane + r.delta * (ptrdiff_t)TS;
+    const uint32_t offx = cx > 0 ? l.om : (cx < 0 ? l.op : l.o0);
+    t[i] = ld_at<T>(base, r.wall ? l.o0 : offx);
+    if (cx != 0) lid[i] = (r.wall && r.top) ? (float)(6.0 * L::w(i) * cx) * a.ulid : 0.0f;
+  }
+  if (PAT != PYLBM_AA_EVEN && (l.wall_lo || l.wall_hi)) {
+    T tw[Q];
+#pragma unroll
+    for (int i = 1; i < Q; ++i)
+      if (L::c(i, 0) != 0) tw[i] = ld_at<T>(Ab + ((PAT == PYLBM_AB) ? L::opp(i) : i) * plane, l.o0);
+#pragma unroll
+    for (int i = 1; i < Q; ++i) {
+      // an x-wall link bounces back; if it also crossed a y/z wall, t[i] already holds the
+      // same own-slot value
+      if ((L::c(i, 0) > 0 && l.wall_lo) || (L::c(i, 0) < 0 && l.wall_hi)) t[i] = tw[i];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < Q; ++i) h[i] = C::ld(t[i]) + (C::shifted ? 0.0f : -(float)L::w(i)) + lid[i];
+}
+
+}  // namespace plbm

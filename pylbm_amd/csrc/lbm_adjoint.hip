@@ -21,6 +21,7 @@
 // a constant and drops out.  Per-cell force and rate sensitivities:
 //   dL/dF_b(x) += B_b / (2 rho) + sum_i lam''_i w_i (3 (c_ib - u_b) + 9 cu_i c_ib)
 //   dL/ds+-    += -sum_i lam+-_i (g_i - feq_i + S_i / 2)      (lam+- = P+- lam)
+#include "lbm_access.hpp"
 #include "lbm_adjoint.hpp"
 
 namespace plbm {
@@ -114,75 +115,58 @@ __global__ void __launch_bounds__(kBlock)
 k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f,
           const float* __restrict__ lam_in, float* __restrict__ lam_out,
           float* __restrict__ dF, double* __restrict__ part) {
-  using C = Codec<ST>;
+  using T = typename Codec<ST>::T;
   constexpr int Q = L::Q;
-  const int b = (int)blockIdx.x;
-  const int seg = b % a.nseg;
-  const int row = b / a.nseg;
-  const int y = row % a.ny;
-  const int z = row / a.ny;
-  const int x = seg * kBlock + (int)threadIdx.x;
-  const bool live = x < a.nx;
+  constexpr uint32_t TS = sizeof(T);
+  Lane l;
+  const bool live = lane_setup<TS>(a, remap_block((int)blockIdx.x, a.nblk, a.remap), l);
   float dsp = 0.f, dsm = 0.f;
   if (live) {
-    const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const int n = x + nx * (y + ny * z);
     const size_t N = (size_t)a.N;
-    const bool per[3] = {(a.per & 1) != 0, ((a.per >> 1) & 1) != 0, ((a.per >> 2) & 1) != 0};
-    const int dims[3] = {nx, ny, nz};
-    const int crd[3] = {x, y, z};
-    float g[Q], lam[Q];
+    const char* fb = reinterpret_cast<const char*>(f);
+    const char* lib = reinterpret_cast<const char*>(lam_in);
+    char* lob = reinterpret_cast<char*>(lam_out);
+    const size_t lplane = N * sizeof(float);
+    const uint32_t q0 = l.n * 4u, qm = l.nm * 4u, qp = l.np * 4u;  // fp32 lane offsets
+    float h[Q], lam[Q];
+    gather<L, ST, PYLBM_AB>(a, l, fb, N * TS, h);
 #pragma unroll
-    for (int i = 0; i < Q; ++i) {
-      int s[3] = {x, y, z};
-      bool wall = false, top = false;
-#pragma unroll
-      for (int d = 0; d < L::D; ++d) {
-        const int c = L::c(i, d);
-        if (c == 0) continue;
-        s[d] = crd[d] - c;
-        if (s[d] < 0 || s[d] >= dims[d]) {
-          if (per[d]) s[d] += (s[d] < 0) ? dims[d] : -dims[d];
-          else { wall = true; if (d == L::D - 1 && s[d] >= dims[d]) top = true; }
-        }
-      }
-      const int src = s[0] + nx * (s[1] + ny * s[2]);
-      const size_t idx = wall ? (size_t)L::opp(i) * N + n : (size_t)i * N + src;
-      // deviation h = g - w: the shifted storages hold it, fp32 subtracts exactly
-      float v = C::shifted ? C::ld(f[idx]) : C::ld(f[idx]) - (float)L::w(i);
-      if (L::c(i, 0) != 0 && top) v += (float)(6.0 * L::w(i) * L::c(i, 0)) * a.ulid;
-      g[i] = v;
-      lam[i] = lam_in[(size_t)i * N + n];
-    }
+    for (int i = 0; i < Q; ++i) lam[i] = ld_at<float>(lib + i * lplane, q0);
     float F[3] = {a.fx, a.fy, a.fz};
+    const char* ffb = reinterpret_cast<const char*>(a.ff);
     if (FF) {
 #pragma unroll
-      for (int d = 0; d < L::D; ++d) F[d] = a.ff[(size_t)d * N + n];
+      for (int d = 0; d < L::D; ++d) F[d] = ld_at<float>(ffb + d * lplane, q0);
     }
     float mu[Q], dFn[3];
-    adjoint_node<L, COLL>(g, F, lam, a.r0, a.r1, mu, dFn, dsp, dsm);
-    // push mu_k back along the link it was pulled over
+    adjoint_node<L, COLL>(h, F, lam, a.r0, a.r1, mu, dFn, dsp, dsm);
+    // push mu_i back along the link it was pulled over: to (i, x - c_i), or to the own
+    // slot opp(i) on a wall link
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      int s[3] = {x, y, z};
-      bool wall = false;
-#pragma unroll
-      for (int d = 0; d < L::D; ++d) {
-        const int c = L::c(i, d);
-        if (c == 0) continue;
-        s[d] = crd[d] - c;
-        if (s[d] < 0 || s[d] >= dims[d]) {
-          if (per[d]) s[d] += (s[d] < 0) ? dims[d] : -dims[d];
-          else wall = true;
-        }
+      if (i == 0) {
+        st_at<2>(lob, q0, mu[0]);
+        continue;
       }
-      const size_t idx = wall ? (size_t)L::opp(i) * N + n
-                              : (size_t)i * N + (s[0] + nx * (s[1] + ny * s[2]));
-      lam_out[idx] = mu[i];
+      const int cx = L::c(i, 0);
+      const RowLink r = row_link<L>(a, l.y, l.z, i, -1);
+      char* own = lob + L::opp(i) * lplane;
+      if (r.wall) {
+        st_at<2>(own, q0, mu[i]);
+        continue;
+      }
+      char* base = lob + i * lplane + r.delta * (ptrdiff_t)4;
+      const uint32_t off = cx > 0 ? qm : (cx < 0 ? qp : q0);
+      if ((cx > 0 && l.wall_lo) || (cx < 0 && l.wall_hi)) st_at<2>(own, q0, mu[i]);
+      else st_at<2>(base, off, mu[i]);
     }
     if (FF) {
+      char* dFb = reinterpret_cast<char*>(dF);
 #pragma unroll
-      for (int d = 0; d < L::D; ++d) dF[(size_t)d * N + n] += dFn[d];
+      for (int d = 0; d < L::D; ++d) {
+        const float v = ld_at<float>(dFb + d * lplane, q0);
+        st_at<0>(dFb + d * lplane, q0, v + dFn[d]);
+      }
     }
   }
   // per-workgroup partial sums of dL/ds+ and dL/ds- (deterministic: one slot per group)
@@ -201,8 +185,9 @@ k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f,
       s0 += red[0][k];
       s1 += red[1][k];
     }
-    part[2 * (size_t)b] += s0;
-    part[2 * (size_t)b + 1] += s1;
+    const size_t b = blockIdx.x;
+    part[2 * b] += s0;
+    part[2 * b + 1] += s1;
   }
 }
 

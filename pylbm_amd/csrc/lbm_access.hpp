@@ -48,6 +48,42 @@ __device__ __forceinline__ void st_at(char* base, uint32_t off, T v) {
                                           AUX);
 }
 
+// Access to element (plane slot, node) of a state array: base + slot * plane + delta + off
+// with delta a uniform byte offset (the row shift of a link) and off the lane's byte offset.
+// SMALL (the whole array spans less than 4 GiB): one descriptor for the array, and the
+// uniform part slot * plane + delta (non-negative: |delta| < plane for every slot >= 1)
+// goes into the scalar soffset operand, so no descriptor is rebuilt per plane.  Otherwise
+// each access gets a descriptor at its plane-and-row base.
+template <class T, bool SMALL>
+__device__ __forceinline__ T ld_pl(const char* A, size_t plane, int slot, ptrdiff_t delta,
+                                   uint32_t off) {
+  if constexpr (SMALL) {
+    const uint32_t so = (uint32_t)(slot * plane + delta);
+    if constexpr (sizeof(T) == 4)
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rsrc_at(A), off, so, 0));
+    else
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(rsrc_at(A), off, so, 0));
+  } else {
+    return ld_at<T>(A + slot * plane + delta, off);
+  }
+}
+
+template <int AUX, bool SMALL, class T>
+__device__ __forceinline__ void st_pl(char* A, size_t plane, int slot, ptrdiff_t delta,
+                                      uint32_t off, T v) {
+  if constexpr (SMALL) {
+    const uint32_t so = (uint32_t)(slot * plane + delta);
+    if constexpr (sizeof(T) == 4)
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), rsrc_at(A), off, so,
+                                            AUX);
+    else
+      __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, v), rsrc_at(A), off, so,
+                                            AUX);
+  } else {
+    st_at<AUX>(A + slot * plane + delta, off, v);
+  }
+}
+
 // A lane's node and the element indices of its own node (n) and its x-1 / x+1 neighbours
 // in the same row (nm, np), with the periodic wrap applied; on an x wall the neighbour
 // index is the own node (a valid placeholder) and wall_lo / wall_hi is set.  o0 / om / op
@@ -119,7 +155,7 @@ __device__ __forceinline__ RowLink row_link(const KArgs& a, int y, int z, int i,
 //   AB      : read slot i at x - c_i, or own slot opp(i) on a wall link
 //   AA odd  : read slot opp(i) at x - c_i, or own slot i on a wall link
 //   AA even : read own slot i
-template <class L, int ST, int PAT>
+template <class L, int ST, int PAT, bool SMALL>
 __device__ __forceinline__ void gather(const KArgs& a, const Lane& l, const char* Ab,
                                        size_t plane, float (&h)[L::Q]) {
   using C = Codec<ST>;
@@ -136,20 +172,21 @@ __device__ __forceinline__ void gather(const KArgs& a, const Lane& l, const char
     lid[i] = 0.0f;
     if (PAT == PYLBM_AA_EVEN || i == 0) {
       const int slot = (PAT == PYLBM_AA_ODD) ? L::opp(i) : i;
-      t[i] = ld_at<T>(Ab + slot * plane, l.o0);
+      t[i] = ld_pl<T, SMALL>(Ab, plane, slot, 0, l.o0);
       continue;
     }
     const RowLink r = row_link<L>(a, l.y, l.z, i, -1);
-    const char* base = r.wall ? Ab + own_bb * plane : Ab + far * plane + r.delta * (ptrdiff_t)TS;
     const uint32_t offx = cx > 0 ? l.om : (cx < 0 ? l.op : l.o0);
-    t[i] = ld_at<T>(base, r.wall ? l.o0 : offx);
+    t[i] = ld_pl<T, SMALL>(Ab, plane, r.wall ? own_bb : far,
+                           r.wall ? 0 : r.delta * (ptrdiff_t)TS, r.wall ? l.o0 : offx);
     if (cx != 0) lid[i] = (r.wall && r.top) ? (float)(6.0 * L::w(i) * cx) * a.ulid : 0.0f;
   }
   if (PAT != PYLBM_AA_EVEN && (l.wall_lo || l.wall_hi)) {
     T tw[Q];
 #pragma unroll
     for (int i = 1; i < Q; ++i)
-      if (L::c(i, 0) != 0) tw[i] = ld_at<T>(Ab + ((PAT == PYLBM_AB) ? L::opp(i) : i) * plane, l.o0);
+      if (L::c(i, 0) != 0)
+        tw[i] = ld_pl<T, SMALL>(Ab, plane, (PAT == PYLBM_AB) ? L::opp(i) : i, 0, l.o0);
 #pragma unroll
     for (int i = 1; i < Q; ++i) {
       // an x-wall link bounces back; if it also crossed a y/z wall, t[i] already holds the

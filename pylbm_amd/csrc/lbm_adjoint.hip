@@ -31,8 +31,14 @@ namespace {
 // Per-node forward quantities and the transposed collision Jacobian.  h_i = g_i - w_i is
 // the deviation of the pulled population from its rest weight (exact in fp32 for g near
 // w), and every difference the Jacobian needs (g - feq, feq - w) is formed from
-// deviations, so the small non-equilibrium part is not lost to cancellation.
-template <class L, int COLL>
+// deviations, so the small non-equilibrium part is not lost to cancellation.  With
+// feq_i = w_i rho (1 + poly_i), poly_i = 3 cu_i + 4.5 cu_i^2 - 1.5 u^2:
+//   A    = sum_i lam'_i feq_i / rho = sum_i lam'_i w_i (1 + poly_i)
+//   B_a  = 3 rho (sum_i lam'_i w_i c_ia (1 + 3 cu_i) - u_a sum_i lam'_i w_i)
+//          [+ 9 sum_i lam''_i w_i cf_i c_ia - 3 F_a sum_i lam''_i w_i   with a force]
+//   mu_k = lam_k - lam'_k + (A - B.u / rho) + sum_a c_ka B_a / rho
+// FORCE = false drops the Guo terms at compile time.
+template <class L, int COLL, bool FORCE>
 __device__ __forceinline__ void adjoint_node(const float (&h)[L::Q], const float F[3],
                                              const float (&lam)[L::Q], float sp, float sm,
                                              float (&mu)[L::Q], float dF[3], float& dsp,
@@ -47,15 +53,18 @@ __device__ __forceinline__ void adjoint_node(const float (&h)[L::Q], const float
       if (L::c(i, a)) j[a] += L::c(i, a) * h[i];
   }
   const float rho = 1.0f + drho;
-  const float inv = 1.0f / rho;
+  const float inv = __builtin_amdgcn_rcpf(rho);
   float u[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-  for (int a = 0; a < D; ++a) u[a] = (j[a] + 0.5f * F[a]) * inv;
+  for (int a = 0; a < D; ++a) u[a] = (FORCE ? j[a] + 0.5f * F[a] : j[a]) * inv;
   const float usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-  const float uf = u[0] * F[0] + u[1] * F[1] + u[2] * F[2];
+  const float uf = FORCE ? u[0] * F[0] + u[1] * F[1] + u[2] * F[2] : 0.0f;
   const float gp = 1.0f - 0.5f * sp, gm = 1.0f - 0.5f * sm;
 
-  float A = 0.f, B[3] = {0.f, 0.f, 0.f}, dFs[3] = {0.f, 0.f, 0.f};
+  float A = 0.f, P = 0.f, Pp = 0.f;            // sum lam' w (1 + poly), sum lam' w, sum lam'' w
+  float Bc[3] = {0.f, 0.f, 0.f};              // sum lam' w c_a (1 + 3 cu)
+  float Sc[3] = {0.f, 0.f, 0.f};              // sum lam'' w c_a cf        (force)
+  float Dc[3] = {0.f, 0.f, 0.f};              // sum lam'' w c_a (3 + 9 cu) (force)
   dsp = 0.f;
   dsm = 0.f;
   float lp_[Q];
@@ -65,26 +74,34 @@ __device__ __forceinline__ void adjoint_node(const float (&h)[L::Q], const float
     const float lplus = 0.5f * (lam[i] + lam[o]);
     const float lminus = 0.5f * (lam[i] - lam[o]);
     const float lp = (COLL == PYLBM_BGK) ? sp * lam[i] : sp * lplus + sm * lminus;
-    const float lpp = (COLL == PYLBM_BGK) ? gp * lam[i] : gp * lplus + gm * lminus;
     lp_[i] = lp;
     const float w = (float)L::w(i);
     const float cu = L::c(i, 0) * u[0] + L::c(i, 1) * u[1] + L::c(i, 2) * u[2];
-    const float cf = L::c(i, 0) * F[0] + L::c(i, 1) * F[1] + L::c(i, 2) * F[2];
-    const float poly = cu * (3.0f + 4.5f * cu) - 1.5f * usq;   // feq = w rho (1 + poly)
-    const float feq_dev = w * (drho + rho * poly);             // feq - w
-    const float S = w * (3.0f * (cf - uf) + 9.0f * cu * cf);
-    A += lp * (w + feq_dev);
-    const float wr = w * rho;
+    const float poly = cu * (3.0f + 4.5f * cu) - 1.5f * usq;
+    const float lw = lp * w;
+    A += lw * (1.0f + poly);
+    P += lw;
+    const float t = lw * (1.0f + 3.0f * cu);
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-      const float ca = (float)L::c(i, a);
-      // dfeq_i/du_a = w rho (3 c_a + 9 cu c_a - 3 u_a);  dS_i/du_a = w (-3 F_a + 9 c_a cf)
-      B[a] += lp * wr * (3.0f * ca + 9.0f * cu * ca - 3.0f * u[a]) +
-              lpp * w * (-3.0f * F[a] + 9.0f * ca * cf);
-      // dS_i/dF_a at fixed u = w (3 (c_a - u_a) + 9 cu c_a)
-      dFs[a] += lpp * w * (3.0f * (ca - u[a]) + 9.0f * cu * ca);
+      if (L::c(i, a) > 0) Bc[a] += t;
+      else if (L::c(i, a) < 0) Bc[a] -= t;
     }
-    const float r = (h[i] - feq_dev) + 0.5f * S;  // g - feq + S/2
+    float S = 0.0f;
+    if (FORCE) {
+      const float lpp = (COLL == PYLBM_BGK) ? gp * lam[i] : gp * lplus + gm * lminus;
+      const float lppw = lpp * w;
+      const float cf = L::c(i, 0) * F[0] + L::c(i, 1) * F[1] + L::c(i, 2) * F[2];
+      S = w * (3.0f * (cf - uf) + 9.0f * cu * cf);
+      Pp += lppw;
+      const float tc = lppw * cf, td = lppw * (3.0f + 9.0f * cu);
+#pragma unroll
+      for (int a = 0; a < D; ++a) {
+        if (L::c(i, a) > 0) { Sc[a] += tc; Dc[a] += td; }
+        else if (L::c(i, a) < 0) { Sc[a] -= tc; Dc[a] -= td; }
+      }
+    }
+    const float r = (h[i] - w * (drho + rho * poly)) + 0.5f * S;  // g - feq + S/2
     if (COLL == PYLBM_BGK) {
       dsp -= lam[i] * r;
     } else {
@@ -92,16 +109,29 @@ __device__ __forceinline__ void adjoint_node(const float (&h)[L::Q], const float
       dsm -= lminus * r;
     }
   }
-  A *= inv;
+  float B[3], Bn[3];
+  float Bu = 0.f;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    B[a] = 3.0f * rho * (Bc[a] - u[a] * P);
+    if (FORCE) B[a] += 9.0f * Sc[a] - 3.0f * F[a] * Pp;
+    Bn[a] = B[a] * inv;
+    Bu += Bn[a] * u[a];
+  }
+  const float K0 = A - Bu;
 #pragma unroll
   for (int k = 0; k < Q; ++k) {
-    float m = lam[k] - lp_[k] + A;
+    float m = lam[k] - lp_[k] + K0;
 #pragma unroll
-    for (int a = 0; a < D; ++a) m += B[a] * ((float)L::c(k, a) - u[a]) * inv;
+    for (int a = 0; a < D; ++a) {
+      if (L::c(k, a) > 0) m += Bn[a];
+      else if (L::c(k, a) < 0) m -= Bn[a];
+    }
     mu[k] = m;
   }
 #pragma unroll
-  for (int a = 0; a < D; ++a) dF[a] = 0.5f * B[a] * inv + dFs[a];
+  for (int a = 0; a < D; ++a)
+    dF[a] = FORCE ? 0.5f * Bn[a] + (Dc[a] - 3.0f * u[a] * Pp) : 0.0f;
 }
 
 __device__ __forceinline__ float wave_sum_f(float v) {
@@ -110,7 +140,7 @@ __device__ __forceinline__ float wave_sum_f(float v) {
   return v;
 }
 
-template <class L, int COLL, int ST, bool FF>
+template <class L, int COLL, int ST, int FM, bool SMALL>
 __global__ void __launch_bounds__(kBlock)
 k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f,
           const float* __restrict__ lam_in, float* __restrict__ lam_out,
@@ -129,43 +159,44 @@ k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f,
     const size_t lplane = N * sizeof(float);
     const uint32_t q0 = l.n * 4u, qm = l.nm * 4u, qp = l.np * 4u;  // fp32 lane offsets
     float h[Q], lam[Q];
-    gather<L, ST, PYLBM_AB>(a, l, fb, N * TS, h);
+    gather<L, ST, PYLBM_AB, SMALL>(a, l, fb, N * TS, h);
 #pragma unroll
-    for (int i = 0; i < Q; ++i) lam[i] = ld_at<float>(lib + i * lplane, q0);
+    for (int i = 0; i < Q; ++i) lam[i] = ld_pl<float, SMALL>(lib, lplane, i, 0, q0);
+    constexpr bool FF = FM == 2;
     float F[3] = {a.fx, a.fy, a.fz};
     const char* ffb = reinterpret_cast<const char*>(a.ff);
     if (FF) {
 #pragma unroll
-      for (int d = 0; d < L::D; ++d) F[d] = ld_at<float>(ffb + d * lplane, q0);
+      for (int d = 0; d < L::D; ++d) F[d] = ld_pl<float, SMALL>(ffb, lplane, d, 0, q0);
     }
     float mu[Q], dFn[3];
-    adjoint_node<L, COLL>(h, F, lam, a.r0, a.r1, mu, dFn, dsp, dsm);
+    adjoint_node<L, COLL, FM != 0>(h, F, lam, a.r0, a.r1, mu, dFn, dsp, dsm);
     // push mu_i back along the link it was pulled over: to (i, x - c_i), or to the own
     // slot opp(i) on a wall link
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       if (i == 0) {
-        st_at<2>(lob, q0, mu[0]);
+        st_pl<2, SMALL>(lob, lplane, 0, 0, q0, mu[0]);
         continue;
       }
       const int cx = L::c(i, 0);
       const RowLink r = row_link<L>(a, l.y, l.z, i, -1);
-      char* own = lob + L::opp(i) * lplane;
       if (r.wall) {
-        st_at<2>(own, q0, mu[i]);
+        st_pl<2, SMALL>(lob, lplane, L::opp(i), 0, q0, mu[i]);
         continue;
       }
-      char* base = lob + i * lplane + r.delta * (ptrdiff_t)4;
       const uint32_t off = cx > 0 ? qm : (cx < 0 ? qp : q0);
-      if ((cx > 0 && l.wall_lo) || (cx < 0 && l.wall_hi)) st_at<2>(own, q0, mu[i]);
-      else st_at<2>(base, off, mu[i]);
+      if ((cx > 0 && l.wall_lo) || (cx < 0 && l.wall_hi))
+        st_pl<2, SMALL>(lob, lplane, L::opp(i), 0, q0, mu[i]);
+      else
+        st_pl<2, SMALL>(lob, lplane, i, r.delta * (ptrdiff_t)4, off, mu[i]);
     }
     if (FF) {
       char* dFb = reinterpret_cast<char*>(dF);
 #pragma unroll
       for (int d = 0; d < L::D; ++d) {
-        const float v = ld_at<float>(dFb + d * lplane, q0);
-        st_at<0>(dFb + d * lplane, q0, v + dFn[d]);
+        const float v = ld_pl<float, SMALL>(dFb, lplane, d, 0, q0);
+        st_pl<0, SMALL>(dFb, lplane, d, 0, q0, v + dFn[d]);
       }
     }
   }
@@ -272,10 +303,27 @@ int adj_step_t(const KArgs& a, bool field, const void* f, const float* lin, floa
                float* dF, double* part, hipStream_t s) {
   using T = typename Codec<ST>::T;
   const dim3 grid((unsigned)a.nblk), block(kBlock);
-  if (field)
-    k_adjoint<L, COLL, ST, true><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF, part);
-  else
-    k_adjoint<L, COLL, ST, false><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF, part);
+  const bool uniform = a.fx != 0.0f || a.fy != 0.0f || a.fz != 0.0f;
+  // one descriptor per array when the adjoint state (fp32, the largest array) fits 4 GiB
+  const bool small = (uint64_t)L::Q * (uint64_t)a.N * 4u < ((uint64_t)1 << 32);
+  if (small) {
+    if (field)
+      k_adjoint<L, COLL, ST, 2, true><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF, part);
+    else if (uniform)
+      k_adjoint<L, COLL, ST, 1, true><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF, part);
+    else
+      k_adjoint<L, COLL, ST, 0, true><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF, part);
+  } else {
+    if (field)
+      k_adjoint<L, COLL, ST, 2, false><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF,
+                                                              part);
+    else if (uniform)
+      k_adjoint<L, COLL, ST, 1, false><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF,
+                                                              part);
+    else
+      k_adjoint<L, COLL, ST, 0, false><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF,
+                                                              part);
+  }
   return (int)hipGetLastError();
 }
 

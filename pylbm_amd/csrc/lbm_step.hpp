@@ -17,7 +17,7 @@ namespace plbm {
 
 enum ForceMode { kForceNone = 0, kForceUniform = 1, kForceField = 2 };
 
-template <class L, int COLL, int ST, int PAT, int FM>
+template <class L, int COLL, int ST, int PAT, int FM, bool SMALL>
 __global__ void __launch_bounds__(kBlock)
 k_step(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T* B) {
   using C = Codec<ST>;
@@ -33,7 +33,7 @@ k_step(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T* B) 
   const size_t plane = N * TS;
 
   float h[Q];  // deviations f - w of the pulled populations
-  gather<L, ST, PAT>(a, l, Ab, plane, h);
+  gather<L, ST, PAT, SMALL>(a, l, Ab, plane, h);
 
   float F[3] = {0.f, 0.f, 0.f};
   if (FM == kForceUniform) {
@@ -55,44 +55,58 @@ k_step(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T* B) 
   for (int i = 0; i < Q; ++i) {
     const float v = C::shifted ? d[i] : d[i] + (float)L::w(i);
     if (PAT == PYLBM_AB) {
-      st_at<2>(Bb + i * plane, l.o0, C::st(v));
+      st_pl<2, SMALL>(Bb, plane, i, 0, l.o0, C::st(v));
       continue;
     }
     if (PAT == PYLBM_AA_EVEN) {
-      st_at<2>(Bb + L::opp(i) * plane, l.o0, C::st(v));
+      st_pl<2, SMALL>(Bb, plane, L::opp(i), 0, l.o0, C::st(v));
       continue;
     }
     if (i == 0) {
-      st_at<0>(Bb, l.o0, C::st(v));
+      st_pl<0, SMALL>(Bb, plane, 0, 0, l.o0, C::st(v));
       continue;
     }
     // AA odd: push to x + c_i, or bounce back into the own slot opp(i)
     const int cx = L::c(i, 0);
     const RowLink r = row_link<L>(a, l.y, l.z, i, +1);
-    char* own = Bb + L::opp(i) * plane;
     if (r.wall) {
       float vb = v;
       if (cx != 0 && r.top) vb = v + (float)(-6.0 * L::w(i) * cx) * a.ulid;
-      st_at<0>(own, l.o0, C::st(vb));
+      st_pl<0, SMALL>(Bb, plane, L::opp(i), 0, l.o0, C::st(vb));
       continue;
     }
-    char* base = Bb + i * plane + r.delta * (ptrdiff_t)TS;
     const uint32_t off = cx > 0 ? l.op : (cx < 0 ? l.om : l.o0);  // push target x + c_x
-    if ((cx > 0 && l.wall_hi) || (cx < 0 && l.wall_lo)) st_at<0>(own, l.o0, C::st(v));
-    else st_at<0>(base, off, C::st(v));
+    if ((cx > 0 && l.wall_hi) || (cx < 0 && l.wall_lo))
+      st_pl<0, SMALL>(Bb, plane, L::opp(i), 0, l.o0, C::st(v));
+    else
+      st_pl<0, SMALL>(Bb, plane, i, r.delta * (ptrdiff_t)TS, off, C::st(v));
   }
+}
+
+template <class L, int COLL, int ST, int PAT, bool SMALL>
+void launch_step_f(const KArgs& a, int fmode, const void* src, void* dst, hipStream_t s) {
+  using T = typename Codec<ST>::T;
+  const dim3 grid((unsigned)a.nblk), block(kBlock);
+  if (fmode == kForceField)
+    k_step<L, COLL, ST, PAT, kForceField, SMALL><<<grid, block, 0, s>>>(a, (const T*)src,
+                                                                        (T*)dst);
+  else if (fmode == kForceUniform)
+    k_step<L, COLL, ST, PAT, kForceUniform, SMALL><<<grid, block, 0, s>>>(a, (const T*)src,
+                                                                          (T*)dst);
+  else
+    k_step<L, COLL, ST, PAT, kForceNone, SMALL><<<grid, block, 0, s>>>(a, (const T*)src,
+                                                                       (T*)dst);
 }
 
 template <class L, int COLL, int ST, int PAT>
 int launch_step_t(const KArgs& a, int fmode, const void* src, void* dst, hipStream_t s) {
-  using T = typename Codec<ST>::T;
-  const dim3 grid((unsigned)a.nblk), block(kBlock);
-  if (fmode == kForceField)
-    k_step<L, COLL, ST, PAT, kForceField><<<grid, block, 0, s>>>(a, (const T*)src, (T*)dst);
-  else if (fmode == kForceUniform)
-    k_step<L, COLL, ST, PAT, kForceUniform><<<grid, block, 0, s>>>(a, (const T*)src, (T*)dst);
+  // one descriptor per array when the population array and the force field fit in 4 GiB
+  const uint64_t bytes = (uint64_t)L::Q * (uint64_t)a.N * sizeof(typename Codec<ST>::T);
+  const uint64_t fbytes = (uint64_t)L::D * (uint64_t)a.N * sizeof(float);
+  if (bytes < ((uint64_t)1 << 32) && fbytes < ((uint64_t)1 << 32))
+    launch_step_f<L, COLL, ST, PAT, true>(a, fmode, src, dst, s);
   else
-    k_step<L, COLL, ST, PAT, kForceNone><<<grid, block, 0, s>>>(a, (const T*)src, (T*)dst);
+    launch_step_f<L, COLL, ST, PAT, false>(a, fmode, src, dst, s);
   return (int)hipGetLastError();
 }
 

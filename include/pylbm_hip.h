@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PYLBM_HIP_ABI_VERSION 1
+#define PYLBM_HIP_ABI_VERSION 2
 
 enum { PYLBM_D2Q9 = 0, PYLBM_D3Q19 = 1 };
 enum { PYLBM_BGK = 0, PYLBM_TRT = 1, PYLBM_MRT = 2 };
@@ -61,6 +61,16 @@ typedef struct pylbm_params {
   float force[3];       /* uniform Guo body force (ignored when force_field != NULL) */
   float rates[4];       /* r0..r3, see above */
   const float* force_field; /* NULL, or per-cell Guo force, SoA (d, N) fp32, device */
+  /*
+   * General geometry (pylbm/problem.py:37-40, semantics pylbm/core.py:56-148).  The
+   * reference's fused kernels take box problems only (pylbm/problem.py:90-92); here the
+   * two-buffer pattern (PYLBM_AB) also runs these, with arrays under 4 GiB in total.
+   */
+  const void* links;    /* NULL, or the per-node solid-source mask of pylbm_hip_links */
+  const float* inlet;   /* NULL, or u_x(y) of the velocity inlet on the x = 0 wall (ny
+                           fp32 values; D2Q9; no effect when x is periodic) */
+  int32_t outlet;       /* 1: rho = 1 pressure outlet on the x = nx - 1 column (D2Q9) */
+  int32_t reserved;     /* must be 0 */
 } pylbm_params;
 
 /* ABI version (PYLBM_HIP_ABI_VERSION of the library). */
@@ -140,6 +150,16 @@ int pylbm_hip_macroscopic(const pylbm_params* p, const void* f, double* rho, dou
  */
 int pylbm_hip_reduce(const pylbm_params* p, int kind, const void* f, const void* other,
                      double* out, void* stream);
+
+/*
+ * Per-node solid-source mask of a solid field: links[n] bit i (i >= 1) is set when the
+ * node x - c_i, wrapped periodically on every axis, is solid (solid: N bytes, nonzero =
+ * solid; links: N uint16 for D2Q9, N uint32 for D3Q19).  Pass links in params.links to
+ * step with obstacles.  Replaces the per-direction rolled solid masks of make_geometry
+ * (pylbm/core.py:56-71) and the q*N int64 gather table of the torch gather solvers
+ * (pylbm/backends/torch_backend.py:80-103).
+ */
+int pylbm_hip_links(const pylbm_params* p, const uint8_t* solid, void* links, void* stream);
 
 /*
  * Streaming copy of nbytes (multiple of 16) from src to dst: the measured copy ceiling.

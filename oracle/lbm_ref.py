@@ -13,6 +13,10 @@ It restates, in its own code, the unfused algorithm of the reference:
 * pull streaming with periodic wrap and half-way
   bounce-back, later axes overriding earlier ones,
   Ladd's moving-lid term on the top wall                  pylbm/core.py:88-127
+* general geometry: stationary solid nodes (bounce-back
+  when the pull source is solid), the velocity inlet on
+  the left wall (Ladd term with the inlet profile) and
+  the rho = 1 pressure outlet on the right column         pylbm/core.py:56-78,96-148
 * moments with the Guo half-force shift                   pylbm/core.py:151-169
 * 2nd-order equilibrium                                   pylbm/core.py:172-186
 * Guo source term                                         pylbm/core.py:189-200
@@ -40,6 +44,9 @@ How each part is pinned (see tests/test_oracle.py):
 * full trajectories: golden vectors of the reference's NumPy solver for the seven
   equivalence cases and the 128^2 cavity after 1000 steps, plus analytic viscosity checks;
 * AA semantics: AA pairs == AB steps (exactly, in float64);
+* general geometry: golden stream / trajectory vectors of the reference's NumPy solver
+  (Schaefer-Turek channel, random solids with inlet, outlet and lid, a 3D obstacle) and a
+  brute-force per-node rule;
 * per-cell force field: equals the uniform-force path when the field is constant.
 """
 
@@ -112,10 +119,12 @@ def magic_rate(s_nu: float, magic: float = 3.0 / 16.0) -> float:
 # ----------------------------------------------------------------------------- case
 @dataclass
 class Case:
-    """The subset of a box problem the oracle needs (pylbm/problem.py:15-42).
+    """The subset of a problem the oracle needs (pylbm/problem.py:15-42).
 
     ``force`` is the uniform Guo force, ``force_field`` (d, *array_shape) a per-cell one
-    (mutually exclusive).  ``s_q`` overrides the MRT energy-flux rate.
+    (mutually exclusive).  ``s_q`` overrides the MRT energy-flux rate.  ``solid`` (bool,
+    array layout), ``inlet`` (u_x(y), shape (ny,), 2D) and ``outlet`` (2D) are the general
+    geometry of pylbm/problem.py:37-40.
     """
 
     shape: tuple
@@ -129,6 +138,9 @@ class Case:
     s_eps: float = 1.54
     s_q: float | None = None
     force_field: np.ndarray | None = None
+    solid: np.ndarray | None = None
+    inlet: np.ndarray | None = None
+    outlet: bool = False
     init_rho: np.ndarray | None = None
     init_u: np.ndarray | None = None
     meta: dict = field(default_factory=dict)
@@ -200,9 +212,11 @@ def _face(d: int, axis: int, side: int) -> tuple:
 def stream(f: np.ndarray, case: Case) -> np.ndarray:
     """Pull streaming + half-way bounce-back + Ladd lid (pylbm/core.py:88-127).
 
-    Axis ``a`` of the problem is array axis ``d-1-a``.  Walls are applied axis by axis so
-    that a later axis overrides an earlier one at edges and corners; the lid (top wall of
-    the last axis) therefore wins at the lid corners.
+    Axis ``a`` of the problem is array axis ``d-1-a``.  Solid sources are bounced back
+    first; walls are then applied axis by axis so that a later axis overrides an earlier
+    one at edges and corners (the lid, top wall of the last axis, wins at the lid corners,
+    and a y wall removes the inlet term at the inlet corners); the outlet overrides
+    everything on the last column.
     """
     lat, d = case.lat, case.dim
     axes = tuple(range(d))
@@ -210,6 +224,13 @@ def stream(f: np.ndarray, case: Case) -> np.ndarray:
     for i in range(lat.q):
         sh = tuple(int(lat.c[i, a]) for a in reversed(range(d)))
         out[i] = np.roll(f[i], sh, axis=axes) if any(sh) else f[i]
+    if case.solid is not None:
+        _solid_pull(out, f, case)
+    inlet = None
+    if case.inlet is not None:
+        if d != 2:
+            raise ValueError("the inlet is 2D only")
+        inlet = np.asarray(case.inlet, dtype=np.float64)
     for a in range(d):
         if case.periodic[a]:
             continue
@@ -217,14 +238,52 @@ def stream(f: np.ndarray, case: Case) -> np.ndarray:
             ca = lat.c[i, a]
             if ca == 1:  # source x - c_i lies behind the low wall
                 fc = _face(d, a, 0)
-                out[i][fc] = f[lat.opp[i]][fc]
+                val = f[lat.opp[i]][fc]
+                if a == 0 and inlet is not None:  # velocity inlet: Ladd term, u_x(y)
+                    val = val + 6.0 * lat.w[i] * lat.c[i, 0] * inlet
+                out[i][fc] = val
             elif ca == -1:  # source behind the high wall
                 fc = _face(d, a, -1)
                 val = f[lat.opp[i]][fc]
                 if a == d - 1 and case.u_lid != 0.0 and lat.c[i, 0] != 0:
                     val = val + 6.0 * lat.w[i] * lat.c[i, 0] * case.u_lid
                 out[i][fc] = val
+    if case.outlet:
+        _outlet(out, f, case)
     return out
+
+
+def _solid_pull(out: np.ndarray, f: np.ndarray, case: Case) -> None:
+    """Bounce-back where the pull source x - c_i is a solid node (pylbm/core.py:100-104).
+
+    The source is located with a periodic wrap on every axis (np.roll), also on walled
+    axes, where the wall rule applied afterwards decides the link anyway.  Solid nodes
+    themselves are streamed and collided like any other node.
+    """
+    lat, d = case.lat, case.dim
+    solid = np.asarray(case.solid, dtype=bool)
+    for i in range(1, lat.q):
+        sh = tuple(int(lat.c[i, a]) for a in reversed(range(d)))
+        src_solid = np.roll(solid, sh, axis=tuple(range(d)))
+        out[i] = np.where(src_solid, f[lat.opp[i]], out[i])
+
+
+def _outlet(out: np.ndarray, f: np.ndarray, case: Case) -> None:
+    """Pressure outlet (pylbm/core.py:130-148): on the last column the populations with
+    c_x = -1 become the rho = 1 equilibrium at the velocity of the stored state one column
+    to the left (no force correction)."""
+    lat = case.lat
+    if case.dim != 2:
+        raise ValueError("the outlet is 2D only")
+    col = f[:, :, -2]
+    rho = col.sum(axis=0)
+    ux = (lat.c[:, 0][:, None] * col).sum(axis=0) / rho
+    uy = (lat.c[:, 1][:, None] * col).sum(axis=0) / rho
+    usq = ux * ux + uy * uy
+    for i in range(lat.q):
+        if lat.c[i, 0] == -1:
+            cu = lat.c[i, 0] * ux + lat.c[i, 1] * uy
+            out[i][:, -1] = lat.w[i] * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * usq)
 
 
 # ----------------------------------------------------------------------------- collide

@@ -12,7 +12,7 @@ from pathlib import Path
 from typing import Any
 
 LIB_PATH = Path(__file__).resolve().parent / "libpylbm_hip.so"
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 D2Q9, D3Q19 = 0, 1
 BGK, TRT, MRT = 0, 1, 2
@@ -32,6 +32,7 @@ EXPORTS = (
     "pylbm_hip_macroscopic", "pylbm_hip_reduce", "pylbm_hip_copy",
     "pylbm_hip_adjoint_workspace", "pylbm_hip_adjoint_seed", "pylbm_hip_adjoint_step",
     "pylbm_hip_adjoint_init", "pylbm_hip_run_store", "pylbm_hip_adjoint_run",
+    "pylbm_hip_links",
 )
 
 
@@ -52,6 +53,10 @@ class Params(ctypes.Structure):
         ("force", ctypes.c_float * 3),
         ("rates", ctypes.c_float * 4),
         ("force_field", ctypes.c_void_p),
+        ("links", ctypes.c_void_p),
+        ("inlet", ctypes.c_void_p),
+        ("outlet", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
@@ -96,6 +101,7 @@ def load() -> Any:
         "pylbm_hip_run_store": ([pp, vp, ctypes.POINTER(vp), i64, vp], i32),
         "pylbm_hip_adjoint_run": ([pp, ctypes.POINTER(vp), i64, vp, vp, vp, vp,
                                    ctypes.POINTER(ctypes.c_int), vp], i32),
+        "pylbm_hip_links": ([pp, vp, vp, vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

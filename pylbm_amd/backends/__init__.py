@@ -20,6 +20,7 @@ _ENTRIES: dict[tuple[str, str], str] = {
     ("hip", "fused-bf16"): "hip_backend:HipFusedBF16",
     ("hip", "fused-aa-fp16"): "hip_backend:HipFusedAAFP16",
     ("hip", "fused-aa-bf16"): "hip_backend:HipFusedAABF16",
+    ("hip", "gather"): "hip_backend:HipGather",
 }
 
 COLORS = {"hip": "#e4002b"}

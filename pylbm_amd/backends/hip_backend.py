@@ -10,6 +10,9 @@ native rung instead of CuPy/NVRTC (pylbm/backends/cupy_backend.py:76-122):
   ``step(n)``, instead of one Python launch per step;
 * encode / decode / initial equilibrium / observables run on the device, so a 512^3 D3Q19
   problem never materialises its 20 GB float64 state on the host;
+* the two-buffer variants also run general geometry (solid nodes, velocity inlet, pressure
+  outlet: pylbm/core.py:56-148), which the reference's fused kernels reject; ``gather`` is
+  the fp32 entry the reference's Karman runner asks for;
 * there is no fallback: without the library or a GPU the entries report unavailable and
   ``make`` raises.
 """
@@ -72,14 +75,17 @@ class HipFused(Solver):
 
     @classmethod
     def supports(cls, problem: Problem) -> tuple[bool, str]:
-        ok, why = check_box(problem)
-        if not ok:
-            return ok, why
+        item = 2 if cls.storage in ("f16", "bf16") else 4
+        if not problem.is_box():
+            # obstacles, inlet and outlet: two-buffer kernels, whole array under 4 GiB
+            if cls.aa:
+                return check_box(problem)
+            if problem.lattice.q * problem.n_nodes * item >= 2**32:
+                return False, "with obstacles, inlet or outlet the state must be < 4 GiB"
         if problem.lattice.name not in cls.lattices:
             return False, f"{problem.lattice.name} not implemented"
         if problem.collision not in cls.collisions:
             return False, f"{problem.collision} not implemented"
-        item = 2 if cls.storage in ("f16", "bf16") else 4
         if problem.n_nodes > MAX_NODES or problem.n_nodes * item >= 2**32:
             return False, "a population plane exceeds the kernels' 32-bit offsets"
         return True, ""
@@ -99,7 +105,13 @@ class HipFused(Solver):
                     problem.dim, -1), device=self.device)
         self.A = torch.empty(self.q * self.N, device=self.device, dtype=self.tdtype)
         self.B = None if self.aa else torch.empty_like(self.A)
+        self.links = self.inlet = None
+        if problem.inlet is not None:
+            self.inlet = torch.as_tensor(np.asarray(problem.inlet, dtype=np.float32),
+                                         device=self.device)
         self.params = self._make_params()
+        if problem.solid is not None:
+            self.set_solid(problem.solid)
         if init == "auto":
             self.init_equilibrium(problem.init_rho, problem.init_u)
         elif init != "none":
@@ -126,7 +138,25 @@ class HipFused(Solver):
         for k, r in enumerate(p.runtime_rates()):
             prm.rates[k] = r
         prm.force_field = _ptr(self.force_field)
+        prm.links = _ptr(self.links)
+        prm.inlet = _ptr(self.inlet)
+        prm.outlet = 1 if p.outlet else 0
         return prm
+
+    def set_solid(self, solid: Any) -> None:
+        """Install a solid field (bool, array layout) and build its per-node link mask on
+        the device (pylbm/core.py:56-71)."""
+        st = torch.as_tensor(np.ascontiguousarray(np.asarray(solid, dtype=bool)).astype(np.uint8)
+                             if not torch.is_tensor(solid) else solid.to(torch.uint8),
+                             device=self.device).contiguous()
+        if st.numel() != self.N:
+            raise ValueError(f"solid must have shape {self.problem.array_shape}")
+        dt = torch.int16 if self.q == 9 else torch.int32
+        links = torch.empty(self.N, dtype=dt, device=self.device)
+        _lib.call("pylbm_hip_links", self.pref, _ptr(st), _ptr(links), _stream())
+        torch.cuda.current_stream().synchronize()
+        self.links = links
+        self.params.links = _ptr(links)
 
     def set_force_field(self, field: Any) -> None:
         """Replace the per-cell force (d, *array_shape) without rebuilding the solver.
@@ -223,6 +253,15 @@ class HipFused(Solver):
 
     def synchronize(self) -> None:
         torch.cuda.synchronize()
+
+
+class HipGather(HipFused):
+    """Any geometry the reference's gather solvers run (obstacles, velocity inlet,
+    pressure outlet; pylbm/backends/torch_backend.py:106-137) as one fused gfx950 kernel:
+    the two-buffer kernel plus a per-node solid-source bit mask (no q*N index table)."""
+
+    variant = "gather"
+    description = "hand-written HIP (gfx950), fused pull with obstacles / inlet / outlet, fp32"
 
 
 class HipFusedAA(HipFused):

@@ -19,6 +19,17 @@ extern template int launch_step<D2Q9>(const KArgs&, int, int, int, int, const vo
                                       hipStream_t);
 extern template int launch_step<D3Q19>(const KArgs&, int, int, int, int, const void*, void*,
                                        hipStream_t);
+template <class L>
+int launch_step_geo(const KArgs& a, int coll, int storage, int fmode, const void* src,
+                    void* dst, hipStream_t s);
+template <class L>
+int launch_links(const uint8_t* solid, void* links, int nx, int ny, int nz, hipStream_t s);
+extern template int launch_step_geo<D2Q9>(const KArgs&, int, int, int, const void*, void*,
+                                          hipStream_t);
+extern template int launch_step_geo<D3Q19>(const KArgs&, int, int, int, const void*, void*,
+                                           hipStream_t);
+extern template int launch_links<D2Q9>(const uint8_t*, void*, int, int, int, hipStream_t);
+extern template int launch_links<D3Q19>(const uint8_t*, void*, int, int, int, hipStream_t);
 }  // namespace plbm
 
 using namespace plbm;
@@ -77,8 +88,24 @@ int check_params(const pylbm_params* p, const char* what) {
     return fail(PYLBM_EINVAL, "%s: a moving lid needs a wall on the last axis", what);
   if (p->lattice == PYLBM_D2Q9 && p->force_field == nullptr && p->force[2] != 0.0f)
     return fail(PYLBM_EINVAL, "%s: D2Q9 has no z force", what);
+  if (p->reserved != 0) return fail(PYLBM_EINVAL, "%s: reserved must be 0", what);
+  if (p->outlet != 0 && p->outlet != 1) return fail(PYLBM_EINVAL, "%s: outlet must be 0 or 1", what);
+  if (p->links || p->inlet || p->outlet) {
+    if ((p->inlet || p->outlet) && p->lattice != PYLBM_D2Q9)
+      return fail(PYLBM_ENOTSUP, "%s: the inlet and the outlet are D2Q9 only", what);
+    if (p->outlet && p->nx < 2) return fail(PYLBM_EINVAL, "%s: the outlet needs nx >= 2", what);
+    if (p->aa)
+      return fail(PYLBM_ENOTSUP, "%s: obstacles, inlet and outlet run on the two-buffer "
+                  "layout only", what);
+    const int64_t q = p->lattice == PYLBM_D2Q9 ? 9 : 19;
+    if (q * N * item >= ((int64_t)1 << 32))
+      return fail(PYLBM_ERANGE, "%s: with obstacles, inlet or outlet the population array "
+                  "must span less than 4 GiB", what);
+  }
   return 0;
 }
+
+bool has_geometry(const pylbm_params* p) { return p->links || p->inlet || p->outlet; }
 
 int make_kargs(const pylbm_params* p, KArgs& a, const char* what) {
   a.nx = p->nx;
@@ -105,6 +132,9 @@ int make_kargs(const pylbm_params* p, KArgs& a, const char* what) {
   a.r2 = p->rates[2];
   a.r3 = p->rates[3];
   a.ff = p->force_field;
+  a.links = p->links;
+  a.inlet = p->inlet;
+  a.outlet = p->outlet;
   return 0;
 }
 
@@ -113,6 +143,12 @@ int step_impl(const pylbm_params* p, const KArgs& a, const void* src, void* dst,
   const int fmode = p->force_field != nullptr ? kForceField
                     : (a.fx != 0.0f || a.fy != 0.0f || a.fz != 0.0f) ? kForceUniform
                                                                      : kForceNone;
+  if (has_geometry(p)) {
+    if (pattern != PYLBM_AB) return PYLBM_ENOTSUP;
+    if (p->lattice == PYLBM_D2Q9)
+      return launch_step_geo<D2Q9>(a, p->collision, p->storage, fmode, src, dst, s);
+    return launch_step_geo<D3Q19>(a, p->collision, p->storage, fmode, src, dst, s);
+  }
   if (p->lattice == PYLBM_D2Q9)
     return launch_step<D2Q9>(a, p->collision, p->storage, pattern, fmode, src, dst, s);
   return launch_step<D3Q19>(a, p->collision, p->storage, pattern, fmode, src, dst, s);
@@ -264,6 +300,16 @@ int pylbm_hip_reduce(const pylbm_params* p, int kind, const void* f, const void*
   return hip_fail(launch_aux(kOpReduce, x, (hipStream_t)stream), "pylbm_hip_reduce");
 }
 
+int pylbm_hip_links(const pylbm_params* p, const uint8_t* solid, void* links, void* stream) {
+  if (int e = check_params(p, "pylbm_hip_links")) return e;
+  if (!solid || !links) return fail(PYLBM_EINVAL, "pylbm_hip_links: NULL buffer");
+  if (p->n_nodes == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  const int e = p->lattice == PYLBM_D2Q9 ? launch_links<D2Q9>(solid, links, p->nx, p->ny, p->nz, s)
+                                         : launch_links<D3Q19>(solid, links, p->nx, p->ny, p->nz, s);
+  return hip_fail(e, "pylbm_hip_links");
+}
+
 int pylbm_hip_copy(const void* src, void* dst, int64_t nbytes, void* stream) {
   if (!src || !dst) return fail(PYLBM_EINVAL, "pylbm_hip_copy: NULL buffer");
   if (nbytes < 0 || nbytes % 16)
@@ -284,6 +330,8 @@ static int check_adjoint(const pylbm_params* p, const char* what) {
   if (p->aa) return fail(PYLBM_EINVAL, "%s: the adjoint runs on the two-buffer layout", what);
   if (p->collision == PYLBM_MRT)
     return fail(PYLBM_ENOTSUP, "%s: the adjoint covers BGK and TRT", what);
+  if (has_geometry(p))
+    return fail(PYLBM_ENOTSUP, "%s: the adjoint covers box geometries only", what);
   return 0;
 }
 

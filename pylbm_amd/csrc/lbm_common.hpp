@@ -134,6 +134,10 @@ struct KArgs {
   float fx, fy, fz;
   float r0, r1, r2, r3;
   const float* ff;  // per-cell force (d, N) or nullptr
+  // general geometry (k_step_geo only)
+  const void* links;   // per-node solid-source mask or nullptr
+  const float* inlet;  // u_x(y) of the velocity inlet on the x = 0 wall, or nullptr
+  int outlet;          // 1: pressure outlet on the last column
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs.  With remap, XCD k processes the

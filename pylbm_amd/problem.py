@@ -25,8 +25,10 @@ class Problem:
 
     ``shape`` is (nx, ny) or (nx, ny, nz).  Each axis is periodic or bounded by half-way
     bounce-back walls; the top wall of the last axis moves with ``u_lid`` in +x (Ladd
-    correction, wall density 1).  ``solid`` / ``inlet`` / ``outlet`` describe general
-    geometry, which the fused kernels do not implement (:meth:`is_box`).
+    correction, wall density 1).  ``solid`` (bool, array layout), ``inlet`` (u_x(y) on the
+    x = 0 wall, 2D) and ``outlet`` (rho = 1 on the last column, 2D) describe general
+    geometry (pylbm/problem.py:37-40); the hip two-buffer variants run it, the AA variants
+    take box problems only (:meth:`is_box`).
     """
 
     shape: tuple[int, ...]
@@ -70,6 +72,20 @@ class Problem:
             if any(v != 0.0 for v in self.force):
                 raise ValueError("give either a uniform force or a force_field, not both")
             self.force_field = ff
+        if self.solid is not None:
+            sol = np.asarray(self.solid, dtype=bool)
+            if sol.shape != self.array_shape:
+                raise ValueError(f"solid must have shape {self.array_shape}")
+            self.solid = sol
+        if self.inlet is not None:
+            if self.dim != 2:
+                raise ValueError("inlet only implemented in 2D")
+            inl = np.asarray(self.inlet, dtype=np.float64)
+            if inl.shape != (self.shape[1],):
+                raise ValueError(f"inlet must have shape ({self.shape[1]},)")
+            self.inlet = inl
+        if self.outlet and self.dim != 2:
+            raise ValueError("outlet only implemented in 2D")
         if self.collision not in COLLISIONS:
             raise ValueError(f"collision must be one of {COLLISIONS}")
         if self.collision == "mrt" and self.dim != 2:

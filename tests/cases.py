@@ -53,3 +53,36 @@ def tgv3d_fields(n: int, amp: float = 0.02):
     u[0] = amp * np.sin(x) * np.cos(y) * np.cos(z)
     u[1] = -amp * np.cos(x) * np.sin(y) * np.cos(z)
     return np.ones((n, n, n)), u
+
+
+#: general-geometry stream cases of tests/golden/geometry.npz (solid / inlet stored there)
+GEO_STREAM = {
+    "gs2_all": dict(shape=(7, 6), u_lid=0.07, outlet=True),
+    "gs2_perx": dict(shape=(6, 5), periodic=(True, False), outlet=True),
+    "gs2_pery": dict(shape=(7, 5), periodic=(False, True), outlet=True),
+    "gs2_inlet_lid": dict(shape=(6, 4), u_lid=0.05),
+    "gs3_solid_lid": dict(shape=(5, 4, 6), periodic=(True, False, False), u_lid=0.05),
+    "gs3_solid_per": dict(shape=(4, 5, 3), periodic=(True, True, True)),
+}
+
+#: general-geometry trajectories of tests/golden/geometry.npz (60 steps each)
+GEO_TRAJ = {
+    "geo2d_mrt_force": dict(shape=(24, 16), tau=0.6, collision="mrt", periodic=(True, False),
+                            force=(1e-5, -2e-6)),
+    "geo3d_trt_lid": dict(shape=(12, 10, 8), tau=0.7, collision="trt",
+                          periodic=(True, False, False), force=(1e-5, 0.0, 2e-6), u_lid=0.05),
+}
+
+
+def geo_stream_kwargs(g: dict, name: str) -> dict:
+    kw = dict(GEO_STREAM[name], solid=g[f"{name}/solid"])
+    if f"{name}/inlet" in g:
+        kw["inlet"] = g[f"{name}/inlet"]
+    return kw
+
+
+def karman6_kwargs(g: dict, collision: str) -> dict:
+    """The Schaefer-Turek d = 6 channel of geometry.npz as problem keyword arguments."""
+    return dict(shape=tuple(int(v) for v in g["karman6/shape"]), tau=float(g["karman6/tau"]),
+                collision=collision, periodic=(False, False), solid=g["karman6/solid"],
+                inlet=g["karman6/inlet"], outlet=True, init_u=g["karman6/init_u"])

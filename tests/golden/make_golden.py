@@ -17,6 +17,12 @@ to this script.  What is generated:
   seeded random populations for tiny grids: walls, lid corners, mixed periodicity, uniform
   force, BGK/TRT/MRT, and MRT with every non-conserved rate overridden (the reference does
   this by monkeypatching ``mrt_rate_tuple``, tests/test_lattice.py:45-46).
+* ``geometry.npz`` (``--only geometry`` writes just this file): general geometry
+  (pylbm/core.py:56-148).  Single ``stream`` calls on seeded random populations for tiny
+  grids with random solid nodes, a velocity inlet, the pressure outlet and a lid; the
+  Schaefer-Turek channel of tests/test_equivalence.py:81-92 (d = 6, 60 steps, TRT as in
+  that test, BGK and MRT besides) with its inputs; 2D and 3D obstacle trajectories with
+  force and lid.
 """
 
 from __future__ import annotations
@@ -30,9 +36,84 @@ REF_SRC = Path("/root/reference/src")
 OUT = Path(__file__).resolve().parent
 
 
+def geometry() -> None:
+    from pylbm.backends.numpy_backend import NumpySolver
+    from pylbm.core import Ops, make_geometry, stream
+    from pylbm.problem import Problem
+    from pylbm.validation.karman import schafer_turek_problem
+
+    rng = np.random.default_rng(20261204)
+    out = {}
+
+    def rand_solid(shape_arr, frac):
+        return rng.random(shape_arr) < frac
+
+    stream_cases = {
+        "gs2_all": dict(shape=(7, 6), u_lid=0.07, inlet=True, outlet=True),
+        "gs2_perx": dict(shape=(6, 5), periodic=(True, False), inlet=True, outlet=True),
+        "gs2_pery": dict(shape=(7, 5), periodic=(False, True), inlet=True, outlet=True),
+        "gs2_inlet_lid": dict(shape=(6, 4), u_lid=0.05, inlet=True),
+        "gs3_solid_lid": dict(shape=(5, 4, 6), periodic=(True, False, False), u_lid=0.05),
+        "gs3_solid_per": dict(shape=(4, 5, 3), periodic=(True, True, True)),
+    }
+    for name, kw in stream_cases.items():
+        kw = dict(kw)
+        shape = kw.pop("shape")
+        ashape = tuple(reversed(shape))
+        solid = rand_solid(ashape, 0.25)
+        inlet = 0.05 * rng.random(ashape[-2]) if kw.pop("inlet", False) else None
+        pr = Problem(shape, solid=solid, inlet=inlet, **kw)
+        q = pr.lattice.q
+        f0 = pr.lattice.w.reshape((-1,) + (1,) * pr.dim) * (
+            1.0 + 0.1 * rng.standard_normal((q,) + pr.array_shape))
+        res = np.stack(stream(f0, make_geometry(pr, np.asarray), Ops(np)))
+        out[f"{name}/in"] = f0
+        out[f"{name}/out"] = res
+        out[f"{name}/solid"] = solid
+        if inlet is not None:
+            out[f"{name}/inlet"] = inlet
+    for coll in ("trt", "bgk", "mrt"):
+        pr = schafer_turek_problem(d=6, u_mean=0.05, collision=coll)
+        s = NumpySolver(pr)
+        s.step(60)
+        out[f"karman6_{coll}/f"] = s.populations()
+        if coll == "trt":
+            out["karman6/solid"] = pr.solid
+            out["karman6/inlet"] = pr.inlet
+            out["karman6/init_u"] = pr.init_u
+            out["karman6/tau"] = np.float64(pr.tau)
+            out["karman6/shape"] = np.array(pr.shape)
+        print(f"karman6 {coll}: {out[f'karman6_{coll}/f'].shape}")
+    # 2D obstacle channel, MRT + force; 3D obstacle box, TRT + force + lid
+    solid2 = np.zeros((16, 24), dtype=bool)
+    solid2[5:10, 6:9] = True
+    solid2[2, 15] = True
+    pr = Problem((24, 16), tau=0.6, collision="mrt", periodic=(True, False),
+                 force=(1e-5, -2e-6), solid=solid2)
+    s = NumpySolver(pr)
+    s.step(60)
+    out["geo2d_mrt_force/f"] = s.populations()
+    out["geo2d_mrt_force/solid"] = solid2
+    solid3 = np.zeros((8, 10, 12), dtype=bool)
+    solid3[2:5, 3:6, 4:8] = True
+    solid3[6, 8, 1] = True
+    pr = Problem((12, 10, 8), tau=0.7, collision="trt", periodic=(True, False, False),
+                 force=(1e-5, 0.0, 2e-6), u_lid=0.05, solid=solid3)
+    s = NumpySolver(pr)
+    s.step(60)
+    out["geo3d_trt_lid/f"] = s.populations()
+    out["geo3d_trt_lid/solid"] = solid3
+    np.savez_compressed(OUT / "geometry.npz", **out)
+    print(f"geometry: {len(out)} arrays")
+
+
 def main() -> None:
     sys.dont_write_bytecode = True
     sys.path.insert(0, str(REF_SRC))
+    if "--only" in sys.argv and sys.argv[sys.argv.index("--only") + 1] == "geometry":
+        geometry()
+        return
+    geometry()
     from pylbm.backends.numpy_backend import NumpySolver
     from pylbm.core import Ops, collide, make_geometry, stream
     from pylbm.problem import Problem

@@ -118,7 +118,7 @@ def test_initial_populations_match_oracle():
 
 def test_registry_never_crashes():
     st = B.status()
-    assert len(st) == len(B.entries()) == 6
+    assert len(st) == len(B.entries()) == 7
     for b, v, ok, why in st:
         assert b == "hip"
         assert ok or why  # unavailable entries say why
@@ -129,12 +129,21 @@ def test_unknown_entry_raises_keyerror():
         B.get("hip", "nope")
 
 
-def test_supports_refuses_non_box_and_bad_collision():
-    cls = B.get("hip", "fused")
+def test_supports_geometry_on_two_buffer_variants_only():
     solid = np.zeros((8, 8), dtype=bool)
-    assert not cls.supports(Problem((8, 8), solid=solid))[0]
+    geo = Problem((8, 8), solid=solid, inlet=np.zeros(8), outlet=True)
+    for v in ("fused", "fused-fp16", "fused-bf16", "gather"):
+        assert B.get("hip", v).supports(geo)[0], v
+    for v in ("fused-aa", "fused-aa-fp16", "fused-aa-bf16"):
+        ok, why = B.get("hip", v).supports(geo)
+        assert not ok and "box" in why
+    cls = B.get("hip", "fused")
     assert cls.supports(Problem((8, 8)))[0]
     assert cls.supports(Problem((8, 8), collision="mrt"))[0]
+    # geometry needs the whole state under 4 GiB (single-descriptor addressing)
+    big = Problem((1 << 14, 1 << 13), solid=None, outlet=True)
+    ok, why = cls.supports(big)
+    assert not ok and "4 GiB" in why
 
 
 def test_make_raises_unsupported_without_gpu():
@@ -169,7 +178,10 @@ def test_params_struct_layout():
     assert _lib.Params.ulid.offset == 48
     assert _lib.Params.rates.offset == 64
     assert _lib.Params.force_field.offset == 80
-    assert ctypes.sizeof(_lib.Params) == 88
+    assert _lib.Params.links.offset == 88
+    assert _lib.Params.inlet.offset == 96
+    assert _lib.Params.outlet.offset == 104
+    assert ctypes.sizeof(_lib.Params) == 112
 
 
 def _params(**over) -> _lib.Params:
@@ -191,6 +203,11 @@ def _params(**over) -> _lib.Params:
     (dict(nz=2, n_nodes=256), "nz == 1"),
     (dict(nx=0, n_nodes=0), "sizes"),
     (dict(aa=2), "aa must be"),
+    (dict(reserved=1), "reserved"),
+    (dict(outlet=2), "outlet must be"),
+    (dict(aa=1, links=0x3000), "two-buffer"),
+    (dict(lattice=_lib.D3Q19, nz=2, n_nodes=256, outlet=1), "D2Q9 only"),
+    (dict(nx=1, n_nodes=8, outlet=1), "nx >= 2"),
 ])
 def test_abi_rejects_bad_params(over, needle):
     lib = _lib.load()

@@ -90,6 +90,63 @@ def run_variant(variant: str, problem, init, steps: int, warmup: int, repeats: i
     return rec
 
 
+def cylinder_array_channel(n: int, coll: str = "trt", d: int = 64, pitch: int = 256):
+    """D2Q9 n x n channel: y walls, parabolic velocity inlet (u_max 0.05) at x = 0, pressure
+    outlet at x = n - 1, a staggered array of solid cylinders (diameter d, pitch pitch)."""
+    from pylbm_amd.problem import Problem
+
+    y, x = np.mgrid[0:n, 0:n]
+    row = y // pitch  # cylinder row k: centre y = k pitch + pitch / 2
+    cy = row * pitch + pitch // 2
+    off = (pitch // 2) * (row % 2)  # staggered rows; centres x = off + j pitch, j >= 1
+    cx = off + np.rint((x - off) / pitch).astype(np.int64) * pitch
+    valid = (cx >= pitch + off) & (cx < n - pitch)
+    solid = valid & ((x - cx) ** 2 + (y - cy) ** 2 <= (d / 2) ** 2)
+    yy = np.arange(n) + 0.5
+    inlet = 4.0 * 0.05 * yy * (n - yy) / n ** 2
+    return Problem((n, n), tau=0.6, collision=coll, periodic=(False, False), solid=solid,
+                   inlet=inlet, outlet=True)
+
+
+def sphere_array_box(n: int, d: int = 32, pitch: int = 64):
+    """D3Q19 periodic n^3 box driven by a uniform force through a cubic array of solid
+    spheres (diameter d, pitch pitch)."""
+    from pylbm_amd.problem import Problem
+
+    c = (np.arange(n) % pitch) - pitch / 2 + 0.5
+    r2 = c[:, None, None] ** 2 + c[None, :, None] ** 2 + c[None, None, :] ** 2
+    solid = r2 <= (d / 2) ** 2
+    return Problem((n, n, n), tau=0.6, collision="trt", periodic=(True, True, True),
+                   force=(1e-6, 0.0, 0.0), solid=solid)
+
+
+def geometry_legs(n2: int, steps: int, warmup: int) -> list:
+    """Obstacles + inlet + outlet (k_step_geo): the cylinder-array channel at the headline
+    size and a sphere-array box in 3D; bytes per update include the link mask."""
+    import torch
+
+    out = []
+    p2 = cylinder_array_channel(n2)
+    for v in ("gather", "fused-fp16", "fused-bf16"):
+        r = run_variant(v, p2, (None, None), steps, warmup)
+        r["workload"] = f"cylinder-array channel {n2}^2 (inlet, outlet, " \
+                        f"{100 * p2.solid.mean():.1f}% solid)"
+        r["bytes_per_update_with_mask"] = r["bytes_per_update"] + 2
+        out.append(r)
+        log(f"  geometry D2Q9 {n2}^2 trt hip/{v}: {r['MLUPS']:.0f} MLUPS {r['GBs']:.0f} GB/s")
+    del p2
+    n3 = 256
+    p3 = sphere_array_box(n3)
+    for v in ("gather", "fused-fp16"):
+        r = run_variant(v, p3, (None, None), 100, 10, repeats=2)
+        r["workload"] = f"sphere-array box {n3}^3 ({100 * p3.solid.mean():.1f}% solid)"
+        r["bytes_per_update_with_mask"] = r["bytes_per_update"] + 4
+        out.append(r)
+        log(f"  geometry D3Q19 {n3}^3 trt hip/{v}: {r['MLUPS']:.0f} MLUPS {r['GBs']:.0f} GB/s")
+    torch.cuda.empty_cache()
+    return out
+
+
 def adjoint_leg(name: str, repeats: int = 3) -> dict:
     """Forward+adjoint throughput of a checkpointed differentiable rollout vs forward only.
 
@@ -277,6 +334,7 @@ def main() -> int:
             log(f"  D3Q19 {n3}^3 trt hip/{v}: {r['MLUPS']:.0f} MLUPS {r['GBs']:.0f} GB/s")
         del init3
         torch.cuda.empty_cache()
+        variants += geometry_legs(n, a.steps, a.warmup)
     for r in variants:
         r["pct_copy"] = 100.0 * r["GBs"] / cbw
         r["pct_spec"] = 100.0 * r["GBs"] / SPEC_BANDWIDTH_GBS

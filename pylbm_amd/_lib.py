@@ -80,6 +80,13 @@ def load() -> Any:
     if not LIB_PATH.exists():
         _LOAD_ERROR = f"{LIB_PATH.name} not built (run python -m pylbm_amd.build)"
         raise OSError(_LOAD_ERROR)
+    # torch owns the device memory the kernels work on: load its HIP runtime first so that
+    # the library's libamdhip64 dependency binds to the runtime already in the process (a
+    # second runtime initialised after torch's finds no device)
+    try:
+        import torch  # noqa: F401
+    except ImportError:  # pragma: no cover - a C-only process uses the system runtime
+        pass
     lib = ctypes.CDLL(str(LIB_PATH))
     vp, i32, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
     pp = ctypes.POINTER(Params)

@@ -1,8 +1,8 @@
 """Command line interface: ``python -m pylbm_amd run | validate | bench | backends``.
 
 Mirrors the reference's ``pylbm`` console script (pylbm/cli.py:98-144) for the hip backend:
-``run cavity --backend hip --variant fused|fused-aa|fused-fp16|fused-bf16`` is the
-north-star entry.  (The reference's ``kernels`` subcommand prints DSL source; there is none.)
+``run cavity --backend hip --variant fused|fused-aa|fused-fp16|fused-bf16`` runs the box
+kernels, ``run karman`` the obstacle / inlet / outlet kernel (variant ``gather``).  (The reference's ``kernels`` subcommand prints DSL source; there is none.)
 """
 
 from __future__ import annotations
@@ -28,6 +28,12 @@ def _cmd_run(a: argparse.Namespace) -> None:
         p = Problem.cavity(a.n, re=a.re, u_lid=a.u_lid, collision=a.collision)
     elif a.case == "cavity3d":
         p = Problem.cavity(a.n, re=a.re, u_lid=a.u_lid, collision=a.collision, dim=3)
+    elif a.case == "karman":
+        from .validation.karman import schafer_turek_problem
+
+        p = schafer_turek_problem(d=a.d, collision=a.collision)
+        if a.variant == "fused" and a.backend == "hip":
+            a.variant = "gather"
     else:
         raise SystemExit(f"unknown case {a.case}")
     s = make_solver(p, a.backend, a.variant)
@@ -63,6 +69,15 @@ def _cmd_validate(a: argparse.Namespace) -> None:
     print(f"Poiseuille TRT: L2 {r['L2']:.2e}")
     c = ghia.run_cavity(100, 64 if a.quick else 128, backend="hip", variant=v)
     print(f"Ghia Re=100: converged={c.converged} Linf={c.errors['Linf']:.4f}")
+    if v.startswith("fused-aa"):
+        print("Karman: skipped (the AA variants take box geometries only)")
+        return
+    from .validation import karman
+
+    r = karman.run_karman(d=20 if a.quick else 40, backend="hip",
+                          variant="gather" if v == "fused" else v)
+    print(f"Karman D={r['d']}: St={r['St']:.4f} cd_max={r['cd_max']:.3f} "
+          f"cl_max={r['cl_max']:.3f} (reference intervals {karman.REFERENCE})")
 
 
 def _cmd_bench(a: argparse.Namespace) -> None:
@@ -77,7 +92,8 @@ def main(argv: list[str] | None = None) -> None:
     sp = sub.add_parser("backends", help="list backends and whether they run here")
     sp.set_defaults(func=_cmd_backends)
     sp = sub.add_parser("run", help="run a case and report throughput")
-    sp.add_argument("case", choices=["cavity", "cavity3d"])
+    sp.add_argument("case", choices=["cavity", "cavity3d", "karman"])
+    sp.add_argument("--d", type=int, default=20, help="karman: nodes per cylinder diameter")
     sp.add_argument("--n", type=int, default=256)
     sp.add_argument("--re", type=float, default=100.0)
     sp.add_argument("--u-lid", type=float, default=0.1)

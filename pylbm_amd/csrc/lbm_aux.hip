@@ -191,9 +191,12 @@ template <class L, int ST>
 __global__ void __launch_bounds__(kBlock)
 k_reduce(const typename Codec<ST>::T* __restrict__ A, const typename Codec<ST>::T* __restrict__ B,
          MomArgs m, int kind, double* out) {
-  const int n = blockIdx.x * kBlock + threadIdx.x;
+  // grid-stride over the nodes with a bounded grid, so that one float64 atomic per
+  // workgroup (a few thousand in all) is all that serialises at the target address
   double v = 0.0;
-  if (n < m.N) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t nn = (int64_t)blockIdx.x * kBlock + threadIdx.x; nn < m.N; nn += stride) {
+    const int n = (int)nn;
     if (kind == PYLBM_RED_MASS) {
 #pragma unroll
       for (int i = 0; i < L::Q; ++i) {
@@ -204,7 +207,7 @@ k_reduce(const typename Codec<ST>::T* __restrict__ A, const typename Codec<ST>::
     } else if (kind == PYLBM_RED_KINETIC) {
       double rho, u[3];
       moments_at<L, ST>(A, m, n, rho, u);
-      v = 0.5 * rho * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+      v += 0.5 * rho * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
     } else {
 #pragma unroll
       for (int i = 0; i < L::Q; ++i) {
@@ -272,7 +275,8 @@ int aux_typed(int op, const AuxArgs& x, hipStream_t s) {
       break;
     case kOpReduce:
       k_zero1<<<1, 64, 0, s>>>(x.d_out);
-      k_reduce<L, ST><<<g, b, 0, s>>>((const T*)x.f, (const T*)x.f2, m, x.kind, x.d_out);
+      k_reduce<L, ST><<<dim3(g.x < 4096u ? g.x : 4096u), b, 0, s>>>(
+          (const T*)x.f, (const T*)x.f2, m, x.kind, x.d_out);
       break;
     default:
       return PYLBM_EINVAL;

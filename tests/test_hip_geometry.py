@@ -61,7 +61,7 @@ def geo():
 def _make(variant: str, p: Problem):
     cls = B.get("hip", variant)
     ok, why = cls.available()
-    assert ok, f"hip/{variant} must be available on a GPU box: {why}"
+    assert ok, f"hip/{variant} must be available where a GPU is present: {why}"
     return cls(p)
 
 

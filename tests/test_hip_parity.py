@@ -26,7 +26,7 @@ VARIANTS = [v for _, v in B.entries()]
 def _make(variant: str, p: Problem):
     cls = B.get("hip", variant)
     ok, why = cls.available()
-    assert ok, f"hip/{variant} must be available on a GPU box: {why}"
+    assert ok, f"hip/{variant} must be available where a GPU is present: {why}"
     return cls(p)
 
 

@@ -256,7 +256,7 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint legs")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only the forward+adjoint legs (prints them, no JSON line)")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--n2", type=int, default=4096)
     ap.add_argument("--n3", type=int, default=512)
     a = ap.parse_args()

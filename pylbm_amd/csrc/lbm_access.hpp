@@ -94,13 +94,20 @@ struct Lane {
   bool wall_lo, wall_hi;
 };
 
-template <uint32_t TS>
-__device__ __forceinline__ bool lane_setup(const KArgs& a, int blk, Lane& l) {
+// Row (y, z) and segment index of workgroup blk (uniform: the scalar unit computes it).
+__device__ __forceinline__ void block_row(const KArgs& a, int blk, int& y, int& z, int& seg) {
   const int row = (int)a.div_nseg.div((uint32_t)blk);
-  const int seg = blk - row * a.nseg;
-  l.z = (int)a.div_ny.div((uint32_t)row);
-  l.y = row - l.z * a.ny;
-  l.x = seg * kBlock + (int)threadIdx.x;
+  seg = blk - row * a.nseg;
+  z = (int)a.div_ny.div((uint32_t)row);
+  y = row - z * a.ny;
+}
+
+// Lane of node (x, y, z); false when x lies past the end of the row.
+template <uint32_t TS>
+__device__ __forceinline__ bool lane_at(const KArgs& a, int y, int z, int x, Lane& l) {
+  l.x = x;
+  l.y = y;
+  l.z = z;
   const int nx = a.nx;
   const bool perx = a.per & 1;
   l.n = (uint32_t)(l.x + nx * (l.y + a.ny * l.z));
@@ -120,6 +127,14 @@ __device__ __forceinline__ bool lane_setup(const KArgs& a, int blk, Lane& l) {
   l.om = l.nm * TS;
   l.op = l.np * TS;
   return l.x < nx;
+}
+
+// One node per lane: workgroup blk is the kBlock-node segment seg of its row.
+template <uint32_t TS>
+__device__ __forceinline__ bool lane_setup(const KArgs& a, int blk, Lane& l) {
+  int y, z, seg;
+  block_row(a, blk, y, z, seg);
+  return lane_at<TS>(a, y, z, seg * kBlock + (int)threadIdx.x, l);
 }
 
 // The y/z part of link i seen from row (y, z) towards y + sign c_y, z + sign c_z
@@ -155,15 +170,16 @@ __device__ __forceinline__ RowLink row_link(const KArgs& a, int y, int z, int i,
 //   AB      : read slot i at x - c_i, or own slot opp(i) on a wall link
 //   AA odd  : read slot opp(i) at x - c_i, or own slot i on a wall link
 //   AA even : read own slot i
+// gather = gather_issue (every load, back to back) + gather_finish (x-wall bounce-back in
+// one divergent block, decode); split so that a lane serving several nodes issues the
+// loads of all of them before it waits for any.
 template <class L, int ST, int PAT, bool SMALL>
-__device__ __forceinline__ void gather(const KArgs& a, const Lane& l, const char* Ab,
-                                       size_t plane, float (&h)[L::Q]) {
-  using C = Codec<ST>;
-  using T = typename C::T;
+__device__ __forceinline__ void gather_issue(const KArgs& a, const Lane& l, const char* Ab,
+                                             size_t plane, typename Codec<ST>::T (&t)[L::Q],
+                                             float (&lid)[L::Q]) {
+  using T = typename Codec<ST>::T;
   constexpr int Q = L::Q;
   constexpr uint32_t TS = sizeof(T);
-  T t[Q];
-  float lid[Q];
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     const int cx = L::c(i, 0);
@@ -181,6 +197,15 @@ __device__ __forceinline__ void gather(const KArgs& a, const Lane& l, const char
                            r.wall ? 0 : r.delta * (ptrdiff_t)TS, r.wall ? l.o0 : offx);
     if (cx != 0) lid[i] = (r.wall && r.top) ? (float)(6.0 * L::w(i) * cx) * a.ulid : 0.0f;
   }
+}
+
+template <class L, int ST, int PAT, bool SMALL>
+__device__ __forceinline__ void gather_finish(const Lane& l, const char* Ab, size_t plane,
+                                              typename Codec<ST>::T (&t)[L::Q],
+                                              const float (&lid)[L::Q], float (&h)[L::Q]) {
+  using C = Codec<ST>;
+  using T = typename C::T;
+  constexpr int Q = L::Q;
   if (PAT != PYLBM_AA_EVEN && (l.wall_lo || l.wall_hi)) {
     T tw[Q];
 #pragma unroll
@@ -196,6 +221,15 @@ __device__ __forceinline__ void gather(const KArgs& a, const Lane& l, const char
   }
 #pragma unroll
   for (int i = 0; i < Q; ++i) h[i] = C::ld(t[i]) + (C::shifted ? 0.0f : -(float)L::w(i)) + lid[i];
+}
+
+template <class L, int ST, int PAT, bool SMALL>
+__device__ __forceinline__ void gather(const KArgs& a, const Lane& l, const char* Ab,
+                                       size_t plane, float (&h)[L::Q]) {
+  typename Codec<ST>::T t[L::Q];
+  float lid[L::Q];
+  gather_issue<L, ST, PAT, SMALL>(a, l, Ab, plane, t, lid);
+  gather_finish<L, ST, PAT, SMALL>(l, Ab, plane, t, lid, h);
 }
 
 }  // namespace plbm

@@ -7,8 +7,9 @@
 // in lbm_access.hpp): one lane per node, scalar-unit row logic, shared 32-bit lane offsets,
 // raw buffer access, a branch-free gather, the collision on deviations f - w
 // (lbm_common.hpp), non-temporal stores for the two-buffer and AA-even patterns, and an
-// XCD-aware workgroup order.  One node per lane with natural alignment measured faster on
-// MI355X than 2, 4 or 8 nodes per lane with unaligned 8/16-byte accesses along x.
+// XCD-aware workgroup order.  A lane serves 1, 2 or 4 nodes one workgroup width apart
+// (nodes_per_lane), so every access stays naturally aligned; 2, 4 or 8 adjacent nodes per
+// lane with unaligned 8/16-byte accesses along x measured slower.
 #pragma once
 
 #include "lbm_access.hpp"
@@ -17,40 +18,16 @@ namespace plbm {
 
 enum ForceMode { kForceNone = 0, kForceUniform = 1, kForceField = 2 };
 
-template <class L, int COLL, int ST, int PAT, int FM, bool SMALL>
-__global__ void __launch_bounds__(kBlock)
-k_step(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T* B) {
+// Store the post-collision deviations d of one node for pattern PAT
+// (pylbm/codegen.py:10-16): AB -> B[i, x]; AA even -> A[opp(i), x]; AA odd -> A[i, x + c_i],
+// or the own slot opp(i) on a wall link (minus the lid term through the lid).
+template <class L, int ST, int PAT, bool SMALL>
+__device__ __forceinline__ void store_node(const KArgs& a, const Lane& l, char* Bb,
+                                           size_t plane, const float (&d)[L::Q]) {
   using C = Codec<ST>;
   using T = typename C::T;
   constexpr int Q = L::Q;
   constexpr uint32_t TS = sizeof(T);
-
-  Lane l;
-  if (!lane_setup<TS>(a, remap_block((int)blockIdx.x, a.nblk, a.remap), l)) return;
-  const size_t N = (size_t)a.N;
-  const char* Ab = reinterpret_cast<const char*>(A);
-  char* Bb = reinterpret_cast<char*>(B);
-  const size_t plane = N * TS;
-
-  float h[Q];  // deviations f - w of the pulled populations
-  gather<L, ST, PAT, SMALL>(a, l, Ab, plane, h);
-
-  float F[3] = {0.f, 0.f, 0.f};
-  if (FM == kForceUniform) {
-    F[0] = a.fx;
-    F[1] = a.fy;
-    F[2] = a.fz;
-  } else if (FM == kForceField) {
-    const char* ffb = reinterpret_cast<const char*>(a.ff);
-    const size_t fplane = N * sizeof(float);
-    const uint32_t of = l.n * 4u;
-    F[0] = ld_at<float>(ffb, of);
-    F[1] = ld_at<float>(ffb + fplane, of);
-    F[2] = (L::D == 3) ? ld_at<float>(ffb + 2 * fplane, of) : 0.0f;
-  }
-  float d[Q];
-  collide_node<L, COLL, FM != kForceNone>(h, d, F, a.r0, a.r1, a.r2, a.r3);
-
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     const float v = C::shifted ? d[i] : d[i] + (float)L::w(i);
@@ -83,19 +60,102 @@ k_step(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T* B) 
   }
 }
 
+// NPL nodes per lane: lane t of workgroup (row, seg) serves x = seg * kBlock * NPL +
+// k * kBlock + t for k < NPL, so every wave access stays one contiguous, aligned run of 64
+// elements; the loads of all NPL nodes are issued before the first collision, which
+// doubles the bytes a wave keeps in flight when NPL = 2.
+template <class L, int COLL, int ST, int PAT, int FM, bool SMALL, int NPL>
+__global__ void __launch_bounds__(kBlock)
+k_step(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T* B) {
+  using C = Codec<ST>;
+  using T = typename C::T;
+  constexpr int Q = L::Q;
+  constexpr uint32_t TS = sizeof(T);
+
+  int y, z, seg;
+  block_row(a, remap_block((int)blockIdx.x, a.nblk, a.remap), y, z, seg);
+  Lane l[NPL];
+  bool act[NPL];
+#pragma unroll
+  for (int k = 0; k < NPL; ++k)
+    act[k] = lane_at<TS>(a, y, z, seg * (kBlock * NPL) + k * kBlock + (int)threadIdx.x, l[k]);
+  if (!act[0]) return;
+#pragma unroll
+  for (int k = 1; k < NPL; ++k)
+    if (!act[k]) l[k] = l[0];  // past the row end: re-read node 0, store nothing
+  const size_t N = (size_t)a.N;
+  const char* Ab = reinterpret_cast<const char*>(A);
+  char* Bb = reinterpret_cast<char*>(B);
+  const size_t plane = N * TS;
+
+  T t[NPL][Q];
+  float lid[NPL][Q];
+#pragma unroll
+  for (int k = 0; k < NPL; ++k) gather_issue<L, ST, PAT, SMALL>(a, l[k], Ab, plane, t[k], lid[k]);
+
+#pragma unroll
+  for (int k = 0; k < NPL; ++k) {
+    float h[Q];  // deviations f - w of the pulled populations
+    gather_finish<L, ST, PAT, SMALL>(l[k], Ab, plane, t[k], lid[k], h);
+
+    float F[3] = {0.f, 0.f, 0.f};
+    if (FM == kForceUniform) {
+      F[0] = a.fx;
+      F[1] = a.fy;
+      F[2] = a.fz;
+    } else if (FM == kForceField) {
+      const char* ffb = reinterpret_cast<const char*>(a.ff);
+      const size_t fplane = N * sizeof(float);
+      const uint32_t of = l[k].n * 4u;
+      F[0] = ld_at<float>(ffb, of);
+      F[1] = ld_at<float>(ffb + fplane, of);
+      F[2] = (L::D == 3) ? ld_at<float>(ffb + 2 * fplane, of) : 0.0f;
+    }
+    float d[Q];
+    collide_node<L, COLL, FM != kForceNone>(h, d, F, a.r0, a.r1, a.r2, a.r3);
+    if (k > 0 && !act[k]) break;
+    store_node<L, ST, PAT, SMALL>(a, l[k], Bb, plane, d);
+  }
+}
+
+// Nodes per lane, from A/B timing on MI355X (same box, alternating runs): serving more
+// nodes per lane keeps more bytes in flight per wave, which pays until register pressure
+// cuts occupancy.  D2Q9: 4 for 16-bit storage (+1-3% over 2), 2 for fp32 two-buffer (+4%
+// over 1), 1 for fp32 AA.  D3Q19 (19 populations per node): 2 for 16-bit (+22% on AA over
+// 1; 4 loses 15%), 1 for fp32 (2 loses 2%).
+template <class L, int ST, int PAT>
+constexpr int nodes_per_lane() {
+  constexpr bool half = sizeof(typename Codec<ST>::T) == 2;
+  if constexpr (L::Q == 9) return half ? 4 : (PAT == PYLBM_AB ? 2 : 1);
+  else return half ? 2 : 1;
+}
+
+// The row decomposition for NPL nodes per lane (segments of kBlock * NPL nodes).
+inline KArgs with_npl(const KArgs& a, int npl) {
+  if (npl == 1) return a;
+  KArgs b = a;
+  b.nseg = (a.nx + kBlock * npl - 1) / (kBlock * npl);
+  b.nblk = b.nseg * a.ny * a.nz;
+  b.remap = (b.nblk % 8 == 0) ? 1 : 0;
+  b.div_nseg = make_fastdiv((uint32_t)b.nseg);
+  return b;
+}
+
 template <class L, int COLL, int ST, int PAT, bool SMALL>
-void launch_step_f(const KArgs& a, int fmode, const void* src, void* dst, hipStream_t s) {
+void launch_step_f(const KArgs& a0, int fmode, const void* src, void* dst, hipStream_t s) {
   using T = typename Codec<ST>::T;
+  constexpr int NPL = nodes_per_lane<L, ST, PAT>();
+  const KArgs a = with_npl(a0, NPL);
   const dim3 grid((unsigned)a.nblk), block(kBlock);
   if (fmode == kForceField)
-    k_step<L, COLL, ST, PAT, kForceField, SMALL><<<grid, block, 0, s>>>(a, (const T*)src,
-                                                                        (T*)dst);
+    k_step<L, COLL, ST, PAT, kForceField, SMALL, NPL><<<grid, block, 0, s>>>(a, (const T*)src,
+                                                                             (T*)dst);
   else if (fmode == kForceUniform)
-    k_step<L, COLL, ST, PAT, kForceUniform, SMALL><<<grid, block, 0, s>>>(a, (const T*)src,
-                                                                          (T*)dst);
+    k_step<L, COLL, ST, PAT, kForceUniform, SMALL, NPL><<<grid, block, 0, s>>>(
+        a, (const T*)src, (T*)dst);
   else
-    k_step<L, COLL, ST, PAT, kForceNone, SMALL><<<grid, block, 0, s>>>(a, (const T*)src,
-                                                                       (T*)dst);
+    k_step<L, COLL, ST, PAT, kForceNone, SMALL, NPL><<<grid, block, 0, s>>>(a, (const T*)src,
+                                                                            (T*)dst);
 }
 
 template <class L, int COLL, int ST, int PAT>

@@ -141,3 +141,31 @@ def test_per_cell_force_field_matches_oracle():
             err = np.abs(s.populations() - ref).max()
             print(f"force field {coll} hip/{variant}: {err:.3e}")
             assert err < 3e-6
+
+
+# Rows longer than one workgroup segment, with partial last segments: a lane serves 1, 2 or
+# 4 nodes one workgroup width (256) apart depending on lattice, storage and pattern, so
+# these widths put walls, the lid corner and the periodic wrap on every node slot of a lane.
+WIDE_ROWS = {
+    "d2_cavity_1100": dict(shape=(1100, 7), tau=0.7, u_lid=0.1, collision="trt"),
+    "d2_perx_1030_force": dict(shape=(1030, 6), tau=0.65, periodic=(True, False),
+                               force=(1e-5, 2e-6), collision="mrt"),
+    "d2_periodic_777": dict(shape=(777, 5), tau=0.6, periodic=(True, True), collision="bgk"),
+    "d3_box_600": dict(shape=(600, 3, 4), tau=0.7, u_lid=0.05, collision="trt"),
+}
+
+
+@pytest.mark.parametrize("variant", [v for v in VARIANTS if v != "gather"])
+@pytest.mark.parametrize("case", list(WIDE_ROWS))
+def test_wide_rows_match_oracle(variant, case):
+    kw = dict(WIDE_ROWS[case])
+    rng = np.random.default_rng(7)
+    ashape = tuple(reversed(kw["shape"]))
+    kw["init_u"] = 0.02 * rng.standard_normal((len(ashape),) + ashape)
+    p = Problem(**kw)
+    ref = R.run(R.Case(**kw), 40)
+    s = _make(variant, p)
+    s.step(40)
+    err = np.abs(s.populations() - ref).max()
+    print(f"{case} hip/{variant}: max|df| = {err:.3e}")
+    assert err < (1e-6 if s.storage == "f32" else TOL[s.storage])

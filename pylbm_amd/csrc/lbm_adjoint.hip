@@ -158,17 +158,25 @@ k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f,
     char* lob = reinterpret_cast<char*>(lam_out);
     const size_t lplane = N * sizeof(float);
     const uint32_t q0 = l.n * 4u, qm = l.nm * 4u, qp = l.np * 4u;  // fp32 lane offsets
+    // every load (pulled f, own lambda, force, dF) is issued before the first wait
     float h[Q], lam[Q];
-    gather<L, ST, PYLBM_AB, SMALL>(a, l, fb, N * TS, h);
+    T t[Q];
+    float lid[Q];
+    gather_issue<L, ST, PYLBM_AB, SMALL>(a, l, fb, N * TS, t, lid);
 #pragma unroll
     for (int i = 0; i < Q; ++i) lam[i] = ld_pl<float, SMALL>(lib, lplane, i, 0, q0);
     constexpr bool FF = FM == 2;
     float F[3] = {a.fx, a.fy, a.fz};
     const char* ffb = reinterpret_cast<const char*>(a.ff);
+    char* dFb = reinterpret_cast<char*>(dF);
+    float dF0[3] = {0.f, 0.f, 0.f};
     if (FF) {
 #pragma unroll
       for (int d = 0; d < L::D; ++d) F[d] = ld_pl<float, SMALL>(ffb, lplane, d, 0, q0);
+#pragma unroll
+      for (int d = 0; d < L::D; ++d) dF0[d] = ld_pl<float, SMALL>(dFb, lplane, d, 0, q0);
     }
+    gather_finish<L, ST, PYLBM_AB, SMALL>(l, fb, N * TS, t, lid, h);
     float mu[Q], dFn[3];
     adjoint_node<L, COLL, FM != 0>(h, F, lam, a.r0, a.r1, mu, dFn, dsp, dsm);
     // push mu_i back along the link it was pulled over: to (i, x - c_i), or to the own
@@ -192,12 +200,8 @@ k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f,
         st_pl<2, SMALL>(lob, lplane, i, r.delta * (ptrdiff_t)4, off, mu[i]);
     }
     if (FF) {
-      char* dFb = reinterpret_cast<char*>(dF);
 #pragma unroll
-      for (int d = 0; d < L::D; ++d) {
-        const float v = ld_pl<float, SMALL>(dFb, lplane, d, 0, q0);
-        st_pl<0, SMALL>(dFb, lplane, d, 0, q0, v + dFn[d]);
-      }
+      for (int d = 0; d < L::D; ++d) st_pl<0, SMALL>(dFb, lplane, d, 0, q0, dF0[d] + dFn[d]);
     }
   }
   // per-workgroup partial sums of dL/ds+ and dL/ds- (deterministic: one slot per group)

@@ -19,7 +19,7 @@
 
 #include <type_traits>
 
-#include "lbm_access.hpp"
+#include "lbm_step.hpp"
 
 namespace plbm {
 
@@ -35,27 +35,18 @@ __host__ __device__ constexpr uint32_t x_bits(int sign) {
   return m;
 }
 
-template <class L, int COLL, int ST, int FM>
-__global__ void __launch_bounds__(kBlock)
-k_step_geo(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T* B) {
-  using C = Codec<ST>;
-  using T = typename C::T;
+// Loads of one node: the pulled values (box rules), their additive terms, the mask.
+template <class L, int ST>
+__device__ __forceinline__ void geo_issue(const KArgs& a, const Lane& l, const char* Ab,
+                                          size_t plane, float uin,
+                                          typename Codec<ST>::T (&t)[L::Q], float (&add)[L::Q],
+                                          uint32_t& bb) {
+  using T = typename Codec<ST>::T;
   using M = LinkMask<L>;
   constexpr int Q = L::Q;
   constexpr uint32_t TS = sizeof(T);
-
-  Lane l;
-  if (!lane_setup<TS>(a, remap_block((int)blockIdx.x, a.nblk, a.remap), l)) return;
-  const size_t plane = (size_t)a.N * TS;
-  const char* Ab = reinterpret_cast<const char*>(A);
-  char* Bb = reinterpret_cast<char*>(B);
-
-  uint32_t bb = 0;
+  bb = 0;
   if (a.links) bb = (uint32_t)ld_at<M>(reinterpret_cast<const char*>(a.links), l.n * sizeof(M));
-  const float uin = a.inlet ? a.inlet[l.y] : 0.0f;  // uniform: one row per workgroup
-
-  T t[Q];
-  float add[Q];
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     const int cx = L::c(i, 0);
@@ -73,6 +64,17 @@ k_step_geo(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T*
     if (L::D == 2 && cx == 1 && !r.wall)
       add[i] = l.wall_lo ? (float)(6.0 * L::w(i)) * uin : 0.0f;
   }
+}
+
+// Bounce-back of solid sources and x walls, decode, outlet: the deviations h of one node.
+template <class L, int ST>
+__device__ __forceinline__ void geo_finish(const KArgs& a, const Lane& l, const char* Ab,
+                                           size_t plane, typename Codec<ST>::T (&t)[L::Q],
+                                           const float (&add)[L::Q], uint32_t bb,
+                                           float (&h)[L::Q]) {
+  using C = Codec<ST>;
+  using T = typename C::T;
+  constexpr int Q = L::Q;
   if (l.wall_lo) bb |= x_bits<L>(+1);
   if (l.wall_hi) bb |= x_bits<L>(-1);
   if (bb != 0) {
@@ -83,7 +85,6 @@ k_step_geo(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T*
     for (int i = 1; i < Q; ++i)
       if ((bb >> i) & 1u) t[i] = tw[i];
   }
-  float h[Q];
 #pragma unroll
   for (int i = 0; i < Q; ++i) h[i] = C::ld(t[i]) + (C::shifted ? 0.0f : -(float)L::w(i)) + add[i];
 
@@ -109,26 +110,67 @@ k_step_geo(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T*
       h[i] = (float)L::w(i) * (3.0f * cu + 4.5f * cu * cu - 1.5f * usq);
     }
   }
+}
 
-  float F[3] = {0.f, 0.f, 0.f};
-  if (FM == 1) {
-    F[0] = a.fx;
-    F[1] = a.fy;
-    F[2] = a.fz;
-  } else if (FM == 2) {
-    const char* ffb = reinterpret_cast<const char*>(a.ff);
-    const size_t fplane = (size_t)a.N * sizeof(float);
-    const uint32_t of = l.n * 4u;
-    F[0] = ld_at<float>(ffb, of);
-    F[1] = ld_at<float>(ffb + fplane, of);
-    F[2] = (L::D == 3) ? ld_at<float>(ffb + 2 * fplane, of) : 0.0f;
-  }
-  float d[Q];
-  collide_node<L, COLL, FM != 0>(h, d, F, a.r0, a.r1, a.r2, a.r3);
+// NPL nodes per lane, one workgroup width apart, as k_step (lbm_step.hpp).
+template <class L, int COLL, int ST, int FM, int NPL>
+__global__ void __launch_bounds__(kBlock)
+k_step_geo(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T* B) {
+  using C = Codec<ST>;
+  using T = typename C::T;
+  constexpr int Q = L::Q;
+  constexpr uint32_t TS = sizeof(T);
+
+  int y, z, seg;
+  block_row(a, remap_block((int)blockIdx.x, a.nblk, a.remap), y, z, seg);
+  Lane l[NPL];
+  bool act[NPL];
 #pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    const float v = C::shifted ? d[i] : d[i] + (float)L::w(i);
-    st_pl<2, true>(Bb, plane, i, 0, l.o0, C::st(v));
+  for (int k = 0; k < NPL; ++k)
+    act[k] = lane_at<TS>(a, y, z, seg * (kBlock * NPL) + k * kBlock + (int)threadIdx.x, l[k]);
+  if (!act[0]) return;
+#pragma unroll
+  for (int k = 1; k < NPL; ++k)
+    if (!act[k]) l[k] = l[0];  // past the row end: re-read node 0, store nothing
+  const size_t plane = (size_t)a.N * TS;
+  const char* Ab = reinterpret_cast<const char*>(A);
+  char* Bb = reinterpret_cast<char*>(B);
+  const float uin = a.inlet ? a.inlet[y] : 0.0f;  // uniform: one row per workgroup
+
+  T t[NPL][Q];
+  float add[NPL][Q];
+  uint32_t bb[NPL];
+  float F[NPL][3];
+#pragma unroll
+  for (int k = 0; k < NPL; ++k) {
+    geo_issue<L, ST>(a, l[k], Ab, plane, uin, t[k], add[k], bb[k]);
+    F[k][0] = F[k][1] = F[k][2] = 0.0f;
+    if (FM == 1) {
+      F[k][0] = a.fx;
+      F[k][1] = a.fy;
+      F[k][2] = a.fz;
+    } else if (FM == 2) {
+      const char* ffb = reinterpret_cast<const char*>(a.ff);
+      const size_t fplane = (size_t)a.N * sizeof(float);
+      const uint32_t of = l[k].n * 4u;
+      F[k][0] = ld_at<float>(ffb, of);
+      F[k][1] = ld_at<float>(ffb + fplane, of);
+      if (L::D == 3) F[k][2] = ld_at<float>(ffb + 2 * fplane, of);
+    }
+  }
+
+#pragma unroll
+  for (int k = 0; k < NPL; ++k) {
+    float h[Q];
+    geo_finish<L, ST>(a, l[k], Ab, plane, t[k], add[k], bb[k], h);
+    float d[Q];
+    collide_node<L, COLL, FM != 0>(h, d, F[k], a.r0, a.r1, a.r2, a.r3);
+    if (k > 0 && !act[k]) break;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const float v = C::shifted ? d[i] : d[i] + (float)L::w(i);
+      st_pl<2, true>(Bb, plane, i, 0, l[k].o0, C::st(v));
+    }
   }
 }
 
@@ -151,17 +193,30 @@ k_links(const uint8_t* solid, LinkMask<L>* links, int nx, int ny, int nz, int N)
   links[n] = (LinkMask<L>)m;
 }
 
-template <class L, int COLL, int ST>
-int launch_geo_t(const KArgs& a, int fmode, const void* src, void* dst, hipStream_t s) {
+// Nodes per lane of the geometry kernel: as the box kernel for D2Q9 (on rows at least one
+// segment long); one for D3Q19, where the bounce-back block's extra registers make two
+// nodes per lane 6% slower (sphere-array box, 256^3, fp16).
+template <class L, int COLL, int ST, int NPL>
+int launch_geo_n(const KArgs& a0, int fmode, const void* src, void* dst, hipStream_t s) {
   using T = typename Codec<ST>::T;
+  const KArgs a = with_npl(a0, NPL);
   const dim3 grid((unsigned)a.nblk), block(kBlock);
   if (fmode == 2)
-    k_step_geo<L, COLL, ST, 2><<<grid, block, 0, s>>>(a, (const T*)src, (T*)dst);
+    k_step_geo<L, COLL, ST, 2, NPL><<<grid, block, 0, s>>>(a, (const T*)src, (T*)dst);
   else if (fmode == 1)
-    k_step_geo<L, COLL, ST, 1><<<grid, block, 0, s>>>(a, (const T*)src, (T*)dst);
+    k_step_geo<L, COLL, ST, 1, NPL><<<grid, block, 0, s>>>(a, (const T*)src, (T*)dst);
   else
-    k_step_geo<L, COLL, ST, 0><<<grid, block, 0, s>>>(a, (const T*)src, (T*)dst);
+    k_step_geo<L, COLL, ST, 0, NPL><<<grid, block, 0, s>>>(a, (const T*)src, (T*)dst);
   return (int)hipGetLastError();
+}
+
+template <class L, int COLL, int ST>
+int launch_geo_t(const KArgs& a, int fmode, const void* src, void* dst, hipStream_t s) {
+  constexpr int NPL = L::Q == 9 ? nodes_per_lane<L, ST, PYLBM_AB>() : 1;
+  if constexpr (NPL > 1) {
+    if (a.nx >= kBlock * NPL) return launch_geo_n<L, COLL, ST, NPL>(a, fmode, src, dst, s);
+  }
+  return launch_geo_n<L, COLL, ST, 1>(a, fmode, src, dst, s);
 }
 
 template <class L, int COLL>

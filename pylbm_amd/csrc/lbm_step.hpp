@@ -90,29 +90,31 @@ k_step(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T* B) 
 
   T t[NPL][Q];
   float lid[NPL][Q];
+  float F[NPL][3];
 #pragma unroll
-  for (int k = 0; k < NPL; ++k) gather_issue<L, ST, PAT, SMALL>(a, l[k], Ab, plane, t[k], lid[k]);
+  for (int k = 0; k < NPL; ++k) {
+    gather_issue<L, ST, PAT, SMALL>(a, l[k], Ab, plane, t[k], lid[k]);
+    F[k][0] = F[k][1] = F[k][2] = 0.0f;
+    if (FM == kForceUniform) {
+      F[k][0] = a.fx;
+      F[k][1] = a.fy;
+      F[k][2] = a.fz;
+    } else if (FM == kForceField) {  // issued with the gather, before any wait
+      const char* ffb = reinterpret_cast<const char*>(a.ff);
+      const size_t fplane = N * sizeof(float);
+      const uint32_t of = l[k].n * 4u;
+      F[k][0] = ld_at<float>(ffb, of);
+      F[k][1] = ld_at<float>(ffb + fplane, of);
+      if (L::D == 3) F[k][2] = ld_at<float>(ffb + 2 * fplane, of);
+    }
+  }
 
 #pragma unroll
   for (int k = 0; k < NPL; ++k) {
     float h[Q];  // deviations f - w of the pulled populations
     gather_finish<L, ST, PAT, SMALL>(l[k], Ab, plane, t[k], lid[k], h);
-
-    float F[3] = {0.f, 0.f, 0.f};
-    if (FM == kForceUniform) {
-      F[0] = a.fx;
-      F[1] = a.fy;
-      F[2] = a.fz;
-    } else if (FM == kForceField) {
-      const char* ffb = reinterpret_cast<const char*>(a.ff);
-      const size_t fplane = N * sizeof(float);
-      const uint32_t of = l[k].n * 4u;
-      F[0] = ld_at<float>(ffb, of);
-      F[1] = ld_at<float>(ffb + fplane, of);
-      F[2] = (L::D == 3) ? ld_at<float>(ffb + 2 * fplane, of) : 0.0f;
-    }
     float d[Q];
-    collide_node<L, COLL, FM != kForceNone>(h, d, F, a.r0, a.r1, a.r2, a.r3);
+    collide_node<L, COLL, FM != kForceNone>(h, d, F[k], a.r0, a.r1, a.r2, a.r3);
     if (k > 0 && !act[k]) break;
     store_node<L, ST, PAT, SMALL>(a, l[k], Bb, plane, d);
   }
@@ -141,10 +143,9 @@ inline KArgs with_npl(const KArgs& a, int npl) {
   return b;
 }
 
-template <class L, int COLL, int ST, int PAT, bool SMALL>
-void launch_step_f(const KArgs& a0, int fmode, const void* src, void* dst, hipStream_t s) {
+template <class L, int COLL, int ST, int PAT, bool SMALL, int NPL>
+void launch_step_n(const KArgs& a0, int fmode, const void* src, void* dst, hipStream_t s) {
   using T = typename Codec<ST>::T;
-  constexpr int NPL = nodes_per_lane<L, ST, PAT>();
   const KArgs a = with_npl(a0, NPL);
   const dim3 grid((unsigned)a.nblk), block(kBlock);
   if (fmode == kForceField)
@@ -156,6 +157,20 @@ void launch_step_f(const KArgs& a0, int fmode, const void* src, void* dst, hipSt
   else
     k_step<L, COLL, ST, PAT, kForceNone, SMALL, NPL><<<grid, block, 0, s>>>(a, (const T*)src,
                                                                             (T*)dst);
+}
+
+// Several nodes per lane only pay on rows at least one full segment long; shorter rows
+// would leave the extra node slots idle and keep their register cost.
+template <class L, int COLL, int ST, int PAT, bool SMALL>
+void launch_step_f(const KArgs& a, int fmode, const void* src, void* dst, hipStream_t s) {
+  constexpr int NPL = nodes_per_lane<L, ST, PAT>();
+  if constexpr (NPL > 1) {
+    if (a.nx >= kBlock * NPL) {
+      launch_step_n<L, COLL, ST, PAT, SMALL, NPL>(a, fmode, src, dst, s);
+      return;
+    }
+  }
+  launch_step_n<L, COLL, ST, PAT, SMALL, 1>(a, fmode, src, dst, s);
 }
 
 template <class L, int COLL, int ST, int PAT>

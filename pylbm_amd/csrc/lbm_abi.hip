@@ -112,8 +112,21 @@ int make_kargs(const pylbm_params* p, KArgs& a, const char* what) {
   a.ny = p->ny;
   a.nz = p->nz;
   a.N = (int)p->n_nodes;
-  a.nseg = (p->nx + kBlock - 1) / kBlock;
-  const int64_t nblk = (int64_t)a.nseg * p->ny * p->nz;
+  const int64_t rows = (int64_t)p->ny * p->nz;
+  a.rows = (int)rows;
+  int64_t nblk;
+  if (p->nx <= 128) {
+    // short rows: 2 rows of 128 lanes or 4 rows of 64 lanes per workgroup
+    a.sw_log2 = p->nx <= 64 ? 6 : 7;
+    a.rpw_log2 = 8 - a.sw_log2;
+    a.nseg = 1;
+    nblk = (rows + (1 << a.rpw_log2) - 1) >> a.rpw_log2;
+  } else {
+    a.sw_log2 = 8;
+    a.rpw_log2 = 0;
+    a.nseg = (p->nx + kBlock - 1) / kBlock;
+    nblk = (int64_t)a.nseg * rows;
+  }
   if (nblk >= ((int64_t)1 << 31)) return fail(PYLBM_ERANGE, "%s: grid too large", what);
   a.nblk = (int)nblk;
   const int dims = p->lattice == PYLBM_D2Q9 ? 2 : 3;

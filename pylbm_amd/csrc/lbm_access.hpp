@@ -94,12 +94,28 @@ struct Lane {
   bool wall_lo, wall_hi;
 };
 
-// Row (y, z) and segment index of workgroup blk (uniform: the scalar unit computes it).
-__device__ __forceinline__ void block_row(const KArgs& a, int blk, int& y, int& z, int& seg) {
-  const int row = (int)a.div_nseg.div((uint32_t)blk);
-  seg = blk - row * a.nseg;
+// Row (y, z) and segment index of the calling wave in workgroup blk, uniform per wave (the
+// scalar unit computes it).  A workgroup is one segment of kBlock lanes of a row, or, for
+// rows of at most 128 nodes, 2 or 4 whole rows of 128 or 64 lanes each, so that short rows
+// leave no wave idle.  False for the waves of the last workgroup past the final row.
+__device__ __forceinline__ bool block_row(const KArgs& a, int blk, int& y, int& z, int& seg) {
+  int row;
+  if (a.rpw_log2 == 0) {
+    row = (int)a.div_nseg.div((uint32_t)blk);
+    seg = blk - row * a.nseg;
+  } else {
+    row = (blk << a.rpw_log2) +
+          __builtin_amdgcn_readfirstlane((int)threadIdx.x >> a.sw_log2);
+    seg = 0;
+  }
   z = (int)a.div_ny.div((uint32_t)row);
   y = row - z * a.ny;
+  return row < a.rows;
+}
+
+// Lane index within its row segment.
+__device__ __forceinline__ int seg_lane(const KArgs& a) {
+  return (int)threadIdx.x & ((1 << a.sw_log2) - 1);
 }
 
 // Lane of node (x, y, z); false when x lies past the end of the row.
@@ -129,12 +145,12 @@ __device__ __forceinline__ bool lane_at(const KArgs& a, int y, int z, int x, Lan
   return l.x < nx;
 }
 
-// One node per lane: workgroup blk is the kBlock-node segment seg of its row.
+// One node per lane: the lane's node in workgroup blk (false: no node).
 template <uint32_t TS>
 __device__ __forceinline__ bool lane_setup(const KArgs& a, int blk, Lane& l) {
   int y, z, seg;
-  block_row(a, blk, y, z, seg);
-  return lane_at<TS>(a, y, z, seg * kBlock + (int)threadIdx.x, l);
+  if (!block_row(a, blk, y, z, seg)) return false;
+  return lane_at<TS>(a, y, z, seg * kBlock + seg_lane(a), l);
 }
 
 // The y/z part of link i seen from row (y, z) towards y + sign c_y, z + sign c_z

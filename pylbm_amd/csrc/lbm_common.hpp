@@ -127,6 +127,9 @@ struct KArgs {
   int N;       // nodes per plane (< 2^31)
   int nseg;    // workgroups per row along x
   int nblk;    // total workgroups
+  int rows;    // ny * nz
+  int sw_log2;   // log2 of the lanes one row segment takes (8, or 7 / 6 for short rows)
+  int rpw_log2;  // log2 of the rows one workgroup serves (0, or 1 / 2 for short rows)
   int per;     // bit a: axis a periodic
   int remap;   // 1: XCD-aware workgroup -> row-segment mapping
   FastDiv div_nseg, div_ny;

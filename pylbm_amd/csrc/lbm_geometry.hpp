@@ -122,12 +122,12 @@ k_step_geo(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T*
   constexpr uint32_t TS = sizeof(T);
 
   int y, z, seg;
-  block_row(a, remap_block((int)blockIdx.x, a.nblk, a.remap), y, z, seg);
+  if (!block_row(a, remap_block((int)blockIdx.x, a.nblk, a.remap), y, z, seg)) return;
   Lane l[NPL];
   bool act[NPL];
 #pragma unroll
   for (int k = 0; k < NPL; ++k)
-    act[k] = lane_at<TS>(a, y, z, seg * (kBlock * NPL) + k * kBlock + (int)threadIdx.x, l[k]);
+    act[k] = lane_at<TS>(a, y, z, seg * (kBlock * NPL) + k * kBlock + seg_lane(a), l[k]);
   if (!act[0]) return;
 #pragma unroll
   for (int k = 1; k < NPL; ++k)

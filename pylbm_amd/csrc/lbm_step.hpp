@@ -73,12 +73,12 @@ k_step(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T* B) 
   constexpr uint32_t TS = sizeof(T);
 
   int y, z, seg;
-  block_row(a, remap_block((int)blockIdx.x, a.nblk, a.remap), y, z, seg);
+  if (!block_row(a, remap_block((int)blockIdx.x, a.nblk, a.remap), y, z, seg)) return;
   Lane l[NPL];
   bool act[NPL];
 #pragma unroll
   for (int k = 0; k < NPL; ++k)
-    act[k] = lane_at<TS>(a, y, z, seg * (kBlock * NPL) + k * kBlock + (int)threadIdx.x, l[k]);
+    act[k] = lane_at<TS>(a, y, z, seg * (kBlock * NPL) + k * kBlock + seg_lane(a), l[k]);
   if (!act[0]) return;
 #pragma unroll
   for (int k = 1; k < NPL; ++k)
@@ -132,10 +132,13 @@ constexpr int nodes_per_lane() {
   else return half ? 2 : 1;
 }
 
-// The row decomposition for NPL nodes per lane (segments of kBlock * NPL nodes).
+// The row decomposition for NPL nodes per lane (segments of kBlock * NPL nodes; only used
+// on rows of at least kBlock * NPL nodes, which one workgroup per segment serves).
 inline KArgs with_npl(const KArgs& a, int npl) {
   if (npl == 1) return a;
   KArgs b = a;
+  b.sw_log2 = 8;
+  b.rpw_log2 = 0;
   b.nseg = (a.nx + kBlock * npl - 1) / (kBlock * npl);
   b.nblk = b.nseg * a.ny * a.nz;
   b.remap = (b.nblk % 8 == 0) ? 1 : 0;

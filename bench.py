@@ -358,7 +358,8 @@ def main() -> int:
     roof = {"bound": "hbm", "achieved": achieved, "peak": SPEC_BANDWIDTH_GBS, "unit": "GB/s",
             "frac": achieved / SPEC_BANDWIDTH_GBS,
             "traffic": traffic["bytes_per_launch"] if traffic else None,
-            "kernel": "k_step<D2Q9,BGK,F32,AB,uniform-force>",
+            "kernel": "k_step<D2Q9, BGK, f32, two-buffer, no force, single descriptor, "
+                      "2 nodes per lane>",
             "algorithmic_bytes_per_launch": head["bytes_per_update"] * n * n,
             "avg_launch_us": t_launch * 1e6,
             "frac_of_measured_copy": achieved / cbw,

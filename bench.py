@@ -11,7 +11,8 @@ that configuration (TRT, MRT, AA, fp16/bf16 shifted storage), the D3Q19 512^3 TR
 Taylor-Green runs (fp32, fp16, bf16; pull and AA), the measured copy ceiling, the roofline
 of the dominant kernel and the float64 NumPy oracle timed on the host (CPU baseline).
 
-``--profile`` runs only the headline kernel and the copy kernel (for rocprofv3).
+``--profile [d2q9|d3q19|d3q19-fp16]`` runs only one step kernel (the headline by default)
+and the copy kernel (for rocprofv3).
 """
 
 from __future__ import annotations
@@ -250,8 +251,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--quick", action="store_true", help="headline + copy ceiling only")
-    ap.add_argument("--profile", action="store_true",
-                    help="only the headline kernel and the copy kernel (for rocprofv3)")
+    ap.add_argument("--profile", nargs="?", const="d2q9", default=None,
+                    choices=["d2q9", "d3q19", "d3q19-fp16"],
+                    help="only one step kernel and the copy kernel, for rocprofv3: the headline "
+                         "(d2q9, default), or D3Q19 512^3 TRT fp32 / fp16")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint legs")
     ap.add_argument("--adjoint-only", action="store_true",
@@ -287,14 +290,25 @@ def main() -> int:
 
     if a.profile:
         from pylbm_amd import backends as B
+        from pylbm_amd.problem import Problem
 
-        p = problem_2d("bgk", n, rho, u)
-        s = B.make("hip", "fused", p, init="none")
-        s.init_equilibrium(rho, u)
+        if a.profile == "d2q9":
+            p = problem_2d("bgk", n, rho, u)
+            s = B.make("hip", "fused", p, init="none")
+            s.init_equilibrium(rho, u)
+            nbytes = 9 * n * n * 4  # the bytes one step reads (and writes)
+        else:
+            n3 = a.n3
+            p = Problem((n3,) * 3, tau=0.55, collision="trt", periodic=(True,) * 3)
+            v = "fused" if a.profile == "d3q19" else "fused-fp16"
+            s = B.make("hip", v, p, init="none")
+            s.init_equilibrium(*tgv_init_device(n3))
+            nbytes = 1 << 30
+        del rho, u
         s.step(a.steps)
         s.synchronize()
-        log(f"profile: {a.steps} steps of hip/fused D2Q9 {n}^2 done")
-        nbytes = 2 * 9 * n * n * 4 // 2  # one plane set: the bytes one step reads
+        log(f"profile: {a.steps} steps of hip/{s.variant} {p.lattice.name} "
+            f"{'x'.join(map(str, p.shape))} done")
         x = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
         y = torch.empty_like(x)
         st = int(torch.cuda.current_stream().cuda_stream)

@@ -8,6 +8,7 @@ recorded on torch's current stream, the stream every kernel of this package is l
 
 from __future__ import annotations
 
+import math
 import statistics
 import time
 from typing import Any
@@ -119,28 +120,31 @@ def bench_problem(dim: int, n: int, collision: str = "bgk"):
     return Problem.cavity(n, re=1000.0, u_lid=0.05, collision=collision, dim=dim)
 
 
-def time_solver(solver: Any, min_time: float, repeats: int) -> tuple[list[float], int]:
-    """Per-repeat wall times and steps per repeat (pylbm/benchmark.py:159-180)."""
-    m = solver.step_multiple
-    steps = 2 * m
+def _wall(solver: Any, steps: int) -> float:
+    t0 = time.perf_counter()
     solver.step(steps)
     solver.synchronize()
+    return time.perf_counter() - t0
+
+
+def time_solver(solver: Any, min_time: float, repeats: int) -> tuple[list[float], int]:
+    """Per-repeat wall times and steps per repeat (the protocol of pylbm/benchmark.py:159-180:
+    a warm-up, repeats of at least ``min_time`` seconds, synchronised on both sides).
+
+    The repeat length comes from probes whose step count grows eightfold until one takes a
+    tenth of ``min_time``; the rate it measures then sizes the repeat with 20% margin.
+    """
+    m = solver.step_multiple
+    solver.step(2 * m)  # warm-up
+    solver.synchronize()
+    probe = 2 * m
     while True:
-        t0 = time.perf_counter()
-        solver.step(steps)
-        solver.synchronize()
-        dt = time.perf_counter() - t0
-        if dt >= min_time:
+        dt = _wall(solver, probe)
+        if dt >= 0.1 * min_time:
             break
-        grow = max(2.0, min(10.0, 1.2 * min_time / max(dt, 1e-6)))
-        steps = int(-(-steps * grow // m) * m)
-    times = []
-    for _ in range(repeats):
-        t0 = time.perf_counter()
-        solver.step(steps)
-        solver.synchronize()
-        times.append(time.perf_counter() - t0)
-    return times, steps
+        probe *= 8
+    steps = max(probe, math.ceil(1.2 * min_time / dt * probe / m) * m)
+    return [_wall(solver, steps) for _ in range(repeats)], steps
 
 
 def measure(variant: str, dim: int, n: int, plan: dict, copy_bw: float,

@@ -1,8 +1,11 @@
-"""Backend registry (mirrors pylbm/backends/__init__.py:16-90).
+"""Backend registry: the reference's (backend, variant) lookup (pylbm/backends/__init__.py:16-90)
+for the ``hip`` rung.
 
-Entries are imported lazily; a missing library or GPU only disables the affected entries
-(:func:`status` never raises).  This package carries the ``hip`` rung only; the same
-``("hip", variant) -> "hip_backend:Class"`` lines drop into the reference's ``_ENTRIES``.
+Same contract as the reference: ``entries()`` lists the pairs in order, ``get()`` imports
+the class lazily (KeyError for an unknown pair), ``status()`` reports every entry and never
+raises, ``make()`` refuses an unavailable entry with ``Unsupported``.  The table holds
+(module, class) pairs; the same ``("hip", variant) -> "hip_backend:Class"`` lines drop into
+the reference's ``_ENTRIES``.
 """
 
 from __future__ import annotations
@@ -13,45 +16,46 @@ from typing import Any
 
 from .base import Solver, Unsupported
 
-_ENTRIES: dict[tuple[str, str], str] = {
-    ("hip", "fused"): "hip_backend:HipFused",
-    ("hip", "fused-aa"): "hip_backend:HipFusedAA",
-    ("hip", "fused-fp16"): "hip_backend:HipFusedFP16",
-    ("hip", "fused-bf16"): "hip_backend:HipFusedBF16",
-    ("hip", "fused-aa-fp16"): "hip_backend:HipFusedAAFP16",
-    ("hip", "fused-aa-bf16"): "hip_backend:HipFusedAABF16",
-    ("hip", "gather"): "hip_backend:HipGather",
+_HIP = "hip_backend"
+_TABLE: dict[tuple[str, str], tuple[str, str]] = {
+    ("hip", "fused"): (_HIP, "HipFused"),
+    ("hip", "fused-aa"): (_HIP, "HipFusedAA"),
+    ("hip", "fused-fp16"): (_HIP, "HipFusedFP16"),
+    ("hip", "fused-bf16"): (_HIP, "HipFusedBF16"),
+    ("hip", "fused-aa-fp16"): (_HIP, "HipFusedAAFP16"),
+    ("hip", "fused-aa-bf16"): (_HIP, "HipFusedAABF16"),
+    ("hip", "gather"): (_HIP, "HipGather"),
 }
 
 COLORS = {"hip": "#e4002b"}
 
 
 def entries() -> list[tuple[str, str]]:
-    return list(_ENTRIES)
+    return [*_TABLE]
 
 
 def get(backend: str, variant: str) -> type[Solver]:
-    """Import and return a solver class (KeyError for an unknown pair)."""
-    try:
-        target = _ENTRIES[(backend, variant)]
-    except KeyError as e:
-        raise KeyError(f"unknown backend/variant {backend}/{variant}") from e
-    mod_name, cls_name = target.split(":")
-    with warnings.catch_warnings():
+    """The solver class of an entry, imported on first use (KeyError if not registered)."""
+    where = _TABLE.get((backend, variant))
+    if where is None:
+        raise KeyError(f"no solver registered as {backend}/{variant}; "
+                       f"known: {', '.join('/'.join(e) for e in _TABLE)}")
+    module, cls = where
+    with warnings.catch_warnings():  # an optional dependency's import noise is not ours
         warnings.simplefilter("ignore")
-        mod = importlib.import_module(f"{__name__}.{mod_name}")
-    return getattr(mod, cls_name)
+        return getattr(importlib.import_module(f".{module}", __name__), cls)
+
+
+def _probe(entry: tuple[str, str]) -> tuple[bool, str]:
+    try:
+        return get(*entry).available()
+    except Exception as e:  # noqa: BLE001 - status() reports, it never raises
+        return False, f"{type(e).__name__}: {e}"
 
 
 def status() -> list[tuple[str, str, bool, str]]:
-    out = []
-    for b, v in _ENTRIES:
-        try:
-            ok, why = get(b, v).available()
-        except Exception as e:  # noqa: BLE001 - the registry never crashes
-            ok, why = False, f"{type(e).__name__}: {e}"
-        out.append((b, v, ok, why))
-    return out
+    """(backend, variant, runs here, why not) for every entry."""
+    return [(b, v, *_probe((b, v))) for b, v in _TABLE]
 
 
 def available() -> list[tuple[str, str]]:

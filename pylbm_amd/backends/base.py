@@ -1,9 +1,8 @@
-"""Common solver interface (mirrors pylbm/backends/base.py:18-103).
+"""The solver interface of the reference (pylbm/backends/base.py:18-103).
 
-The canonical state exchanged through :meth:`Solver.populations` /
-:meth:`Solver.set_populations` is the post-collision population array in SoA layout
-``(q, ny, nx)`` or ``(q, nz, ny, nx)`` as float64, whatever the internal layout, storage
-precision or streaming pattern.
+A solver advances one :class:`~pylbm_amd.problem.Problem`.  Whatever it stores internally
+(precision, AA slot order, device), it exchanges the *canonical state*: the post-collision
+populations as a float64 array of shape ``(q, ny, nx)`` or ``(q, nz, ny, nx)``.
 """
 
 from __future__ import annotations
@@ -17,62 +16,59 @@ from ..problem import Problem
 
 
 class Unsupported(Exception):
-    """Raised when a backend/variant cannot run a given problem."""
+    """A backend/variant cannot run the given problem (or cannot run here)."""
 
 
 class Solver(ABC):
-    """A time stepper for one :class:`~pylbm_amd.problem.Problem`."""
-
+    # identity and traffic model, as class attributes (pylbm/backends/base.py:27-35)
     backend: ClassVar[str] = ""
     variant: ClassVar[str] = ""
     description: ClassVar[str] = ""
-    #: storage bytes per population (minimal traffic model: 2 q itemsize per node update)
-    itemsize: ClassVar[int] = 4
-    #: in-place schemes (AA pattern) advance in pairs of steps
-    step_multiple: ClassVar[int] = 1
-    #: does the minimal traffic model hold
-    equivalent: ClassVar[bool] = True
+    itemsize: ClassVar[int] = 4        # bytes per stored population value
+    step_multiple: ClassVar[int] = 1   # AA solvers advance in pairs of steps
+    equivalent: ClassVar[bool] = True  # the 2 q itemsize traffic model applies
 
     def __init__(self, problem: Problem) -> None:
         ok, why = self.supports(problem)
         if not ok:
-            raise Unsupported(f"{self.backend}/{self.variant}: {why}")
+            raise Unsupported(f"{self.name}: {why}")
         self.problem = problem
         self.steps_done = 0
 
     @classmethod
     def available(cls) -> tuple[bool, str]:
+        """(True, "") if the solver can run on this machine, else (False, reason)."""
         return True, ""
 
     @classmethod
     def supports(cls, problem: Problem) -> tuple[bool, str]:
+        """(True, "") if the solver implements ``problem``, else (False, reason)."""
         return True, ""
 
     @abstractmethod
     def set_populations(self, f: np.ndarray) -> None:
-        """Load a canonical post-collision state."""
+        """Replace the state by a canonical one."""
 
     @abstractmethod
     def populations(self) -> np.ndarray:
-        """Return the canonical post-collision state (float64 numpy array)."""
+        """The canonical state as a float64 numpy array."""
+
+    @abstractmethod
+    def macroscopic(self) -> tuple[np.ndarray, np.ndarray]:
+        """rho (array shape) and u (d, *array shape), float64, of the post-collision state."""
 
     @abstractmethod
     def _advance(self, n: int) -> None:
-        """Advance ``n`` steps (a multiple of :attr:`step_multiple`)."""
+        """Run ``n`` steps; ``n`` is a multiple of ``step_multiple``."""
 
     def step(self, n: int = 1) -> None:
-        """Advance ``n`` time steps (pylbm/backends/base.py:68-73)."""
         if n % self.step_multiple:
             raise ValueError(f"{self.name} advances in multiples of {self.step_multiple} steps")
         self._advance(n)
         self.steps_done += n
 
     def synchronize(self) -> None:
-        return None
-
-    @abstractmethod
-    def macroscopic(self) -> tuple[np.ndarray, np.ndarray]:
-        """(rho, u) as float64 numpy arrays; ``u`` has shape (d, *array_shape)."""
+        """Wait for queued work (nothing to wait for on the host)."""
 
     @property
     def name(self) -> str:
@@ -80,15 +76,15 @@ class Solver(ABC):
 
     @property
     def bytes_per_node(self) -> int:
-        """Minimal memory traffic per lattice update: read q + write q populations."""
+        """Minimal traffic of one node update: q populations read, q written."""
         return 2 * self.problem.lattice.q * self.itemsize
 
     def memory_bytes(self) -> int:
-        return 2 * self.problem.lattice.q * self.itemsize * self.problem.n_nodes
+        return self.bytes_per_node * self.problem.n_nodes
 
 
 def check_box(problem: Problem) -> tuple[bool, str]:
-    """Fused kernels implement box problems only (pylbm/backends/base.py:99-103)."""
-    if not problem.is_box():
-        return False, "fused kernels support box geometries only (no obstacles/inlet/outlet)"
-    return True, ""
+    """Box problems only: no solid nodes, inlet or outlet (pylbm/backends/base.py:99-103)."""
+    if problem.is_box():
+        return True, ""
+    return False, "this variant runs box geometries only (no obstacles, inlet or outlet)"

@@ -34,7 +34,10 @@ enum { PYLBM_D2Q9 = 0, PYLBM_D3Q19 = 1 };
 enum { PYLBM_BGK = 0, PYLBM_TRT = 1, PYLBM_MRT = 2 };
 enum { PYLBM_F32 = 0, PYLBM_F16 = 1, PYLBM_BF16 = 2, PYLBM_F32S = 3 };
 enum { PYLBM_AB = 0, PYLBM_AA_ODD = 1, PYLBM_AA_EVEN = 2 };
-enum { PYLBM_RED_MASS = 0, PYLBM_RED_KINETIC = 1, PYLBM_RED_MAXABS_DIFF = 2 };
+enum {
+  PYLBM_RED_MASS = 0, PYLBM_RED_KINETIC = 1, PYLBM_RED_MAXABS_DIFF = 2, PYLBM_RED_LOSS = 3,
+  PYLBM_RED_DOT = 4
+};
 
 enum {
   PYLBM_EINVAL = 1001,   /* bad parameters (message says which) */
@@ -144,7 +147,12 @@ int pylbm_hip_macroscopic(const pylbm_params* p, const void* f, double* rho, dou
  *   PYLBM_RED_MASS        sum over nodes and populations of f
  *   PYLBM_RED_KINETIC     sum over nodes of 0.5 rho |u|^2 (post-collision u)
  *   PYLBM_RED_MAXABS_DIFF max |f - g| over all canonical values, other = second state
- *                         with the same params.
+ *                         with the same params
+ *   PYLBM_RED_LOSS        0.5 sum over nodes of |u - u_target|^2, other = u_target as
+ *                         float64 (d, N) (the rollout loss)
+ *   PYLBM_RED_DOT         sum over all canonical values of f g, other = second state with
+ *                         the same params (with storage PYLBM_F32, any two fp32 (q, N)
+ *                         arrays, e.g. adjoint states)
  * No reference counterpart as a kernel: the reference computes these on the host from
  * populations() (pylbm/validation/taylor_green.py:43-44, tests/test_reference.py:10-15).
  */

@@ -18,7 +18,7 @@ D2Q9, D3Q19 = 0, 1
 BGK, TRT, MRT = 0, 1, 2
 F32, F16, BF16, F32S = 0, 1, 2, 3
 AB, AA_ODD, AA_EVEN = 0, 1, 2
-RED_MASS, RED_KINETIC, RED_MAXABS_DIFF = 0, 1, 2
+RED_MASS, RED_KINETIC, RED_MAXABS_DIFF, RED_LOSS, RED_DOT = 0, 1, 2, 3, 4
 
 LATTICE_IDS = {"D2Q9": D2Q9, "D3Q19": D3Q19}
 COLLISION_IDS = {"bgk": BGK, "trt": TRT, "mrt": MRT}

@@ -224,13 +224,23 @@ class HipFused(Solver):
         rho, u = self.macroscopic_device()
         return rho.cpu().numpy(), u.cpu().numpy()
 
-    def reduce(self, kind: str, other: HipFused | None = None) -> float:
-        """Float64 reduction on the device: 'mass', 'kinetic' or 'maxabs_diff'."""
+    def reduce(self, kind: str, other: Any = None) -> float:
+        """Float64 reduction on the device: 'mass', 'kinetic', 'maxabs_diff' or 'dot' (with
+        ``other`` a solver with the same parameters), or 'loss' (0.5 sum |u - u_target|^2,
+        ``other`` the target velocity (d, *array_shape))."""
         k = {"mass": _lib.RED_MASS, "kinetic": _lib.RED_KINETIC,
-             "maxabs_diff": _lib.RED_MAXABS_DIFF}[kind]
+             "maxabs_diff": _lib.RED_MAXABS_DIFF, "loss": _lib.RED_LOSS,
+             "dot": _lib.RED_DOT}[kind]
+        second = None
+        if kind == "loss":
+            second = torch.as_tensor(other, dtype=torch.float64, device=self.device).contiguous()
+            if second.numel() != self.problem.dim * self.N:
+                raise ValueError("the target velocity has the wrong size")
+            ptr = _ptr(second)
+        else:
+            ptr = _ptr(other.A) if other is not None else None
         out = torch.empty(1, dtype=torch.float64, device=self.device)
-        _lib.call("pylbm_hip_reduce", self.pref, k, _ptr(self.A),
-                  _ptr(other.A) if other is not None else None, _ptr(out), _stream())
+        _lib.call("pylbm_hip_reduce", self.pref, k, _ptr(self.A), ptr, _ptr(out), _stream())
         return float(out.item())
 
     def memory_bytes(self) -> int:

@@ -301,13 +301,16 @@ int pylbm_hip_reduce(const pylbm_params* p, int kind, const void* f, const void*
                      double* out, void* stream) {
   if (int e = check_params(p, "pylbm_hip_reduce")) return e;
   if (!f || !out) return fail(PYLBM_EINVAL, "pylbm_hip_reduce: NULL buffer");
-  if (kind < PYLBM_RED_MASS || kind > PYLBM_RED_MAXABS_DIFF)
+  if (kind < PYLBM_RED_MASS || kind > PYLBM_RED_DOT)
     return fail(PYLBM_EINVAL, "pylbm_hip_reduce: unknown kind %d", kind);
-  if (kind == PYLBM_RED_MAXABS_DIFF && !other)
-    return fail(PYLBM_EINVAL, "pylbm_hip_reduce: MAXABS_DIFF needs a second state");
+  if ((kind == PYLBM_RED_MAXABS_DIFF || kind == PYLBM_RED_DOT) && !other)
+    return fail(PYLBM_EINVAL, "pylbm_hip_reduce: this kind needs a second state");
+  if (kind == PYLBM_RED_LOSS && !other)
+    return fail(PYLBM_EINVAL, "pylbm_hip_reduce: LOSS needs the target velocity");
   AuxArgs x = aux_from(p);
   x.f = f;
-  x.f2 = other;
+  x.f2 = kind == PYLBM_RED_LOSS ? nullptr : other;
+  x.d_in = kind == PYLBM_RED_LOSS ? static_cast<const double*>(other) : nullptr;
   x.kind = kind;
   x.d_out = out;
   return hip_fail(launch_aux(kOpReduce, x, (hipStream_t)stream), "pylbm_hip_reduce");

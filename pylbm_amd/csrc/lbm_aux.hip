@@ -190,7 +190,7 @@ __device__ void block_reduce_to(double v, double* out) {
 template <class L, int ST>
 __global__ void __launch_bounds__(kBlock)
 k_reduce(const typename Codec<ST>::T* __restrict__ A, const typename Codec<ST>::T* __restrict__ B,
-         MomArgs m, int kind, double* out) {
+         const double* __restrict__ target, MomArgs m, int kind, double* out) {
   // grid-stride over the nodes with a bounded grid, so that one float64 atomic per
   // workgroup (a few thousand in all) is all that serialises at the target address
   double v = 0.0;
@@ -208,6 +208,25 @@ k_reduce(const typename Codec<ST>::T* __restrict__ A, const typename Codec<ST>::
       double rho, u[3];
       moments_at<L, ST>(A, m, n, rho, u);
       v += 0.5 * rho * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    } else if (kind == PYLBM_RED_LOSS) {
+      double rho, u[3];
+      moments_at<L, ST>(A, m, n, rho, u);
+#pragma unroll
+      for (int a = 0; a < L::D; ++a) {
+        const double e = u[a] - target[(size_t)a * m.N + n];
+        v += 0.5 * e * e;
+      }
+    } else if (kind == PYLBM_RED_DOT) {
+#pragma unroll
+      for (int i = 0; i < L::Q; ++i) {
+        double a = (double)Codec<ST>::ld(A[(size_t)i * m.N + n]);
+        double b = (double)Codec<ST>::ld(B[(size_t)i * m.N + n]);
+        if (Codec<ST>::shifted) {
+          a += L::w(i);
+          b += L::w(i);
+        }
+        v += a * b;
+      }
     } else {
 #pragma unroll
       for (int i = 0; i < L::Q; ++i) {
@@ -276,7 +295,7 @@ int aux_typed(int op, const AuxArgs& x, hipStream_t s) {
     case kOpReduce:
       k_zero1<<<1, 64, 0, s>>>(x.d_out);
       k_reduce<L, ST><<<dim3(g.x < 4096u ? g.x : 4096u), b, 0, s>>>(
-          (const T*)x.f, (const T*)x.f2, m, x.kind, x.d_out);
+          (const T*)x.f, (const T*)x.f2, x.d_in, m, x.kind, x.d_out);
       break;
     default:
       return PYLBM_EINVAL;

@@ -143,6 +143,28 @@ def test_state_roundtrip_and_observables(variant):
     assert np.array_equal(s.populations(), want)
 
 
+@pytest.mark.parametrize("variant", ["fused", "fused-aa-bf16"])
+def test_loss_and_dot_reductions(variant):
+    """Float64 device reductions LOSS (0.5 sum |u - u_t|^2) and DOT (sum f g over canonical
+    values) against the host formulas on the decoded states."""
+    rng = np.random.default_rng(5)
+    kw = dict(shape=(45, 21), tau=0.7, periodic=(True, False), force=(2.0**-15, 0.0))
+    p = Problem(**kw)
+    case = R.Case(**kw)
+    a, b = _make(variant, p), _make(variant, p)
+    a.init_equilibrium(1 + 0.01 * rng.standard_normal((21, 45)),
+                       0.02 * rng.standard_normal((2, 21, 45)))
+    a.step(6)
+    b.step(4)
+    fa, fb = a.populations(), b.populations()
+    _, ua = R.macroscopic(fa, case)
+    ut = 0.01 * rng.standard_normal((2, 21, 45))
+    want_loss = 0.5 * ((ua - ut) ** 2).sum()
+    assert abs(a.reduce("loss", ut) - want_loss) <= 1e-12 * abs(want_loss)
+    want_dot = (fa * fb).sum()
+    assert abs(a.reduce("dot", b) - want_dot) <= 1e-12 * abs(want_dot)
+
+
 def test_per_cell_force_field_matches_oracle():
     """Per-cell Guo force (extension): HIP vs oracle on a channel, 100 steps."""
     rng = np.random.default_rng(11)

@@ -232,6 +232,8 @@ def test_abi_rejects_bad_calls():
     assert lib.pylbm_hip_run(ctypes.byref(p), a, a, 2, None) == 1001
     assert lib.pylbm_hip_copy(a, b, 17, None) == 1001
     assert lib.pylbm_hip_reduce(ctypes.byref(p), 9, a, None, b, None) == 1001
+    for kind in (_lib.RED_MAXABS_DIFF, _lib.RED_DOT, _lib.RED_LOSS):
+        assert lib.pylbm_hip_reduce(ctypes.byref(p), kind, a, None, b, None) == 1001
     lid = _params(ulid=0.1)
     assert lib.pylbm_hip_step(ctypes.byref(lid), a, b, _lib.AB, None) == 1001
     assert "lid" in lib.pylbm_hip_last_error().decode()

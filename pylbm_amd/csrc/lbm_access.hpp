@@ -49,35 +49,60 @@ __device__ __forceinline__ void st_at(char* base, uint32_t off, T v) {
 }
 
 // Access to element (plane slot, node) of a state array: base + slot * plane + delta + off
-// with delta a uniform byte offset (the row shift of a link) and off the lane's byte offset.
-// SMALL (the whole array spans less than 4 GiB): one descriptor for the array, and the
-// uniform part slot * plane + delta (non-negative: |delta| < plane for every slot >= 1)
-// goes into the scalar soffset operand, so no descriptor is rebuilt per plane.  Otherwise
-// each access gets a descriptor at its plane-and-row base.
-template <class T, bool SMALL>
+// with delta a uniform byte offset (the row shift of a link, |delta| < plane) and off the
+// lane's byte offset.  ADDR chooses how the uniform part reaches the hardware:
+//   1  the whole array spans less than 4 GiB: one descriptor for the array, and
+//      slot * plane + delta (non-negative: |delta| < plane for every slot >= 1, slot 0
+//      never shifts) in the scalar soffset operand;
+//   G  (>= 2) larger arrays: the planes form groups of G slots, group g read through one
+//      descriptor based one plane before its first slot (at plane 0 for g = 0), so that
+//      soffset = (slot - base slot) * plane + delta stays non-negative and soffset + off
+//      stays below (G + 2) planes, which the host keeps under 4 GiB; a kernel then holds
+//      ceil(q / G) descriptors per array instead of one per plane and row;
+//   0  one descriptor per access at its plane-and-row base (any size).
+template <int ADDR>
+__device__ __forceinline__ void plane_ref(const char* A, size_t plane, int slot, ptrdiff_t delta,
+                                          const char*& base, uint32_t& so) {
+  if constexpr (ADDR == 1) {
+    base = A;
+    so = (uint32_t)(slot * plane + delta);
+  } else {
+    const int g = slot / ADDR;
+    const int first = g == 0 ? 0 : g * ADDR - 1;
+    base = A + (size_t)first * plane;
+    so = (uint32_t)((slot - first) * plane + delta);
+  }
+}
+
+template <class T, int ADDR>
 __device__ __forceinline__ T ld_pl(const char* A, size_t plane, int slot, ptrdiff_t delta,
                                    uint32_t off) {
-  if constexpr (SMALL) {
-    const uint32_t so = (uint32_t)(slot * plane + delta);
+  if constexpr (ADDR >= 1) {
+    const char* base;
+    uint32_t so;
+    plane_ref<ADDR>(A, plane, slot, delta, base, so);
     if constexpr (sizeof(T) == 4)
-      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rsrc_at(A), off, so, 0));
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rsrc_at(base), off, so, 0));
     else
-      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(rsrc_at(A), off, so, 0));
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(rsrc_at(base), off, so, 0));
   } else {
     return ld_at<T>(A + slot * plane + delta, off);
   }
 }
 
-template <int AUX, bool SMALL, class T>
+template <int AUX, int ADDR, class T>
 __device__ __forceinline__ void st_pl(char* A, size_t plane, int slot, ptrdiff_t delta,
                                       uint32_t off, T v) {
-  if constexpr (SMALL) {
-    const uint32_t so = (uint32_t)(slot * plane + delta);
+  if constexpr (ADDR >= 1) {
+    const char* base;
+    uint32_t so;
+    plane_ref<ADDR>(A, plane, slot, delta, base, so);
+    char* b = const_cast<char*>(base);
     if constexpr (sizeof(T) == 4)
-      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), rsrc_at(A), off, so,
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), rsrc_at(b), off, so,
                                             AUX);
     else
-      __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, v), rsrc_at(A), off, so,
+      __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, v), rsrc_at(b), off, so,
                                             AUX);
   } else {
     st_at<AUX>(A + slot * plane + delta, off, v);
@@ -189,7 +214,7 @@ __device__ __forceinline__ RowLink row_link(const KArgs& a, int y, int z, int i,
 // gather = gather_issue (every load, back to back) + gather_finish (x-wall bounce-back in
 // one divergent block, decode); split so that a lane serving several nodes issues the
 // loads of all of them before it waits for any.
-template <class L, int ST, int PAT, bool SMALL>
+template <class L, int ST, int PAT, int ADDR>
 __device__ __forceinline__ void gather_issue(const KArgs& a, const Lane& l, const char* Ab,
                                              size_t plane, typename Codec<ST>::T (&t)[L::Q],
                                              float (&lid)[L::Q]) {
@@ -204,18 +229,18 @@ __device__ __forceinline__ void gather_issue(const KArgs& a, const Lane& l, cons
     lid[i] = 0.0f;
     if (PAT == PYLBM_AA_EVEN || i == 0) {
       const int slot = (PAT == PYLBM_AA_ODD) ? L::opp(i) : i;
-      t[i] = ld_pl<T, SMALL>(Ab, plane, slot, 0, l.o0);
+      t[i] = ld_pl<T, ADDR>(Ab, plane, slot, 0, l.o0);
       continue;
     }
     const RowLink r = row_link<L>(a, l.y, l.z, i, -1);
     const uint32_t offx = cx > 0 ? l.om : (cx < 0 ? l.op : l.o0);
-    t[i] = ld_pl<T, SMALL>(Ab, plane, r.wall ? own_bb : far,
+    t[i] = ld_pl<T, ADDR>(Ab, plane, r.wall ? own_bb : far,
                            r.wall ? 0 : r.delta * (ptrdiff_t)TS, r.wall ? l.o0 : offx);
     if (cx != 0) lid[i] = (r.wall && r.top) ? (float)(6.0 * L::w(i) * cx) * a.ulid : 0.0f;
   }
 }
 
-template <class L, int ST, int PAT, bool SMALL>
+template <class L, int ST, int PAT, int ADDR>
 __device__ __forceinline__ void gather_finish(const Lane& l, const char* Ab, size_t plane,
                                               typename Codec<ST>::T (&t)[L::Q],
                                               const float (&lid)[L::Q], float (&h)[L::Q]) {
@@ -227,7 +252,7 @@ __device__ __forceinline__ void gather_finish(const Lane& l, const char* Ab, siz
 #pragma unroll
     for (int i = 1; i < Q; ++i)
       if (L::c(i, 0) != 0)
-        tw[i] = ld_pl<T, SMALL>(Ab, plane, (PAT == PYLBM_AB) ? L::opp(i) : i, 0, l.o0);
+        tw[i] = ld_pl<T, ADDR>(Ab, plane, (PAT == PYLBM_AB) ? L::opp(i) : i, 0, l.o0);
 #pragma unroll
     for (int i = 1; i < Q; ++i) {
       // an x-wall link bounces back; if it also crossed a y/z wall, t[i] already holds the
@@ -239,13 +264,13 @@ __device__ __forceinline__ void gather_finish(const Lane& l, const char* Ab, siz
   for (int i = 0; i < Q; ++i) h[i] = C::ld(t[i]) + (C::shifted ? 0.0f : -(float)L::w(i)) + lid[i];
 }
 
-template <class L, int ST, int PAT, bool SMALL>
+template <class L, int ST, int PAT, int ADDR>
 __device__ __forceinline__ void gather(const KArgs& a, const Lane& l, const char* Ab,
                                        size_t plane, float (&h)[L::Q]) {
   typename Codec<ST>::T t[L::Q];
   float lid[L::Q];
-  gather_issue<L, ST, PAT, SMALL>(a, l, Ab, plane, t, lid);
-  gather_finish<L, ST, PAT, SMALL>(l, Ab, plane, t, lid, h);
+  gather_issue<L, ST, PAT, ADDR>(a, l, Ab, plane, t, lid);
+  gather_finish<L, ST, PAT, ADDR>(l, Ab, plane, t, lid, h);
 }
 
 }  // namespace plbm

@@ -140,7 +140,7 @@ __device__ __forceinline__ float wave_sum_f(float v) {
   return v;
 }
 
-template <class L, int COLL, int ST, int FM, bool SMALL>
+template <class L, int COLL, int ST, int FM, int ADDR>
 __global__ void __launch_bounds__(kBlock)
 k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f,
           const float* __restrict__ lam_in, float* __restrict__ lam_out,
@@ -162,9 +162,9 @@ k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f,
     float h[Q], lam[Q];
     T t[Q];
     float lid[Q];
-    gather_issue<L, ST, PYLBM_AB, SMALL>(a, l, fb, N * TS, t, lid);
+    gather_issue<L, ST, PYLBM_AB, ADDR>(a, l, fb, N * TS, t, lid);
 #pragma unroll
-    for (int i = 0; i < Q; ++i) lam[i] = ld_pl<float, SMALL>(lib, lplane, i, 0, q0);
+    for (int i = 0; i < Q; ++i) lam[i] = ld_pl<float, ADDR>(lib, lplane, i, 0, q0);
     constexpr bool FF = FM == 2;
     float F[3] = {a.fx, a.fy, a.fz};
     const char* ffb = reinterpret_cast<const char*>(a.ff);
@@ -172,11 +172,11 @@ k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f,
     float dF0[3] = {0.f, 0.f, 0.f};
     if (FF) {
 #pragma unroll
-      for (int d = 0; d < L::D; ++d) F[d] = ld_pl<float, SMALL>(ffb, lplane, d, 0, q0);
+      for (int d = 0; d < L::D; ++d) F[d] = ld_pl<float, ADDR>(ffb, lplane, d, 0, q0);
 #pragma unroll
-      for (int d = 0; d < L::D; ++d) dF0[d] = ld_pl<float, SMALL>(dFb, lplane, d, 0, q0);
+      for (int d = 0; d < L::D; ++d) dF0[d] = ld_pl<float, ADDR>(dFb, lplane, d, 0, q0);
     }
-    gather_finish<L, ST, PYLBM_AB, SMALL>(l, fb, N * TS, t, lid, h);
+    gather_finish<L, ST, PYLBM_AB, ADDR>(l, fb, N * TS, t, lid, h);
     float mu[Q], dFn[3];
     adjoint_node<L, COLL, FM != 0>(h, F, lam, a.r0, a.r1, mu, dFn, dsp, dsm);
     // push mu_i back along the link it was pulled over: to (i, x - c_i), or to the own
@@ -184,24 +184,24 @@ k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f,
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       if (i == 0) {
-        st_pl<2, SMALL>(lob, lplane, 0, 0, q0, mu[0]);
+        st_pl<2, ADDR>(lob, lplane, 0, 0, q0, mu[0]);
         continue;
       }
       const int cx = L::c(i, 0);
       const RowLink r = row_link<L>(a, l.y, l.z, i, -1);
       if (r.wall) {
-        st_pl<2, SMALL>(lob, lplane, L::opp(i), 0, q0, mu[i]);
+        st_pl<2, ADDR>(lob, lplane, L::opp(i), 0, q0, mu[i]);
         continue;
       }
       const uint32_t off = cx > 0 ? qm : (cx < 0 ? qp : q0);
       if ((cx > 0 && l.wall_lo) || (cx < 0 && l.wall_hi))
-        st_pl<2, SMALL>(lob, lplane, L::opp(i), 0, q0, mu[i]);
+        st_pl<2, ADDR>(lob, lplane, L::opp(i), 0, q0, mu[i]);
       else
-        st_pl<2, SMALL>(lob, lplane, i, r.delta * (ptrdiff_t)4, off, mu[i]);
+        st_pl<2, ADDR>(lob, lplane, i, r.delta * (ptrdiff_t)4, off, mu[i]);
     }
     if (FF) {
 #pragma unroll
-      for (int d = 0; d < L::D; ++d) st_pl<0, SMALL>(dFb, lplane, d, 0, q0, dF0[d] + dFn[d]);
+      for (int d = 0; d < L::D; ++d) st_pl<0, ADDR>(dFb, lplane, d, 0, q0, dF0[d] + dFn[d]);
     }
   }
   // per-workgroup partial sums of dL/ds+ and dL/ds- (deterministic: one slot per group)
@@ -312,20 +312,20 @@ int adj_step_t(const KArgs& a, bool field, const void* f, const float* lin, floa
   const bool small = (uint64_t)L::Q * (uint64_t)a.N * 4u < ((uint64_t)1 << 32);
   if (small) {
     if (field)
-      k_adjoint<L, COLL, ST, 2, true><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF, part);
+      k_adjoint<L, COLL, ST, 2, 1><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF, part);
     else if (uniform)
-      k_adjoint<L, COLL, ST, 1, true><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF, part);
+      k_adjoint<L, COLL, ST, 1, 1><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF, part);
     else
-      k_adjoint<L, COLL, ST, 0, true><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF, part);
+      k_adjoint<L, COLL, ST, 0, 1><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF, part);
   } else {
     if (field)
-      k_adjoint<L, COLL, ST, 2, false><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF,
+      k_adjoint<L, COLL, ST, 2, 0><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF,
                                                               part);
     else if (uniform)
-      k_adjoint<L, COLL, ST, 1, false><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF,
+      k_adjoint<L, COLL, ST, 1, 0><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF,
                                                               part);
     else
-      k_adjoint<L, COLL, ST, 0, false><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF,
+      k_adjoint<L, COLL, ST, 0, 0><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF,
                                                               part);
   }
   return (int)hipGetLastError();

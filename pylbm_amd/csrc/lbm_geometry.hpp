@@ -52,12 +52,12 @@ __device__ __forceinline__ void geo_issue(const KArgs& a, const Lane& l, const c
     const int cx = L::c(i, 0);
     add[i] = 0.0f;
     if (i == 0) {
-      t[0] = ld_pl<T, true>(Ab, plane, 0, 0, l.o0);
+      t[0] = ld_pl<T, 1>(Ab, plane, 0, 0, l.o0);
       continue;
     }
     const RowLink r = row_link<L>(a, l.y, l.z, i, -1);
     const uint32_t offx = cx > 0 ? l.om : (cx < 0 ? l.op : l.o0);
-    t[i] = ld_pl<T, true>(Ab, plane, r.wall ? L::opp(i) : i,
+    t[i] = ld_pl<T, 1>(Ab, plane, r.wall ? L::opp(i) : i,
                           r.wall ? 0 : r.delta * (ptrdiff_t)TS, r.wall ? l.o0 : offx);
     if (cx != 0 && r.wall && r.top) add[i] = (float)(6.0 * L::w(i) * cx) * a.ulid;
     // inlet: crosses the x = 0 wall and no later (y / z) wall
@@ -80,7 +80,7 @@ __device__ __forceinline__ void geo_finish(const KArgs& a, const Lane& l, const 
   if (bb != 0) {
     T tw[Q];
 #pragma unroll
-    for (int i = 1; i < Q; ++i) tw[i] = ld_pl<T, true>(Ab, plane, L::opp(i), 0, l.o0);
+    for (int i = 1; i < Q; ++i) tw[i] = ld_pl<T, 1>(Ab, plane, L::opp(i), 0, l.o0);
 #pragma unroll
     for (int i = 1; i < Q; ++i)
       if ((bb >> i) & 1u) t[i] = tw[i];
@@ -92,7 +92,7 @@ __device__ __forceinline__ void geo_finish(const KArgs& a, const Lane& l, const 
     // rho = 1 equilibrium at the velocity of the stored column x - 1 (no force correction)
     T tc[Q];
 #pragma unroll
-    for (int i = 0; i < Q; ++i) tc[i] = ld_pl<T, true>(Ab, plane, i, 0, l.om);
+    for (int i = 0; i < Q; ++i) tc[i] = ld_pl<T, 1>(Ab, plane, i, 0, l.om);
     float rho = 1.0f, jx = 0.0f, jy = 0.0f;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
@@ -169,7 +169,7 @@ k_step_geo(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T*
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       const float v = C::shifted ? d[i] : d[i] + (float)L::w(i);
-      st_pl<2, true>(Bb, plane, i, 0, l[k].o0, C::st(v));
+      st_pl<2, 1>(Bb, plane, i, 0, l[k].o0, C::st(v));
     }
   }
 }

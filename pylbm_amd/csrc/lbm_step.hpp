@@ -21,7 +21,7 @@ enum ForceMode { kForceNone = 0, kForceUniform = 1, kForceField = 2 };
 // Store the post-collision deviations d of one node for pattern PAT
 // (pylbm/codegen.py:10-16): AB -> B[i, x]; AA even -> A[opp(i), x]; AA odd -> A[i, x + c_i],
 // or the own slot opp(i) on a wall link (minus the lid term through the lid).
-template <class L, int ST, int PAT, bool SMALL>
+template <class L, int ST, int PAT, int ADDR>
 __device__ __forceinline__ void store_node(const KArgs& a, const Lane& l, char* Bb,
                                            size_t plane, const float (&d)[L::Q]) {
   using C = Codec<ST>;
@@ -32,15 +32,15 @@ __device__ __forceinline__ void store_node(const KArgs& a, const Lane& l, char* 
   for (int i = 0; i < Q; ++i) {
     const float v = C::shifted ? d[i] : d[i] + (float)L::w(i);
     if (PAT == PYLBM_AB) {
-      st_pl<2, SMALL>(Bb, plane, i, 0, l.o0, C::st(v));
+      st_pl<2, ADDR>(Bb, plane, i, 0, l.o0, C::st(v));
       continue;
     }
     if (PAT == PYLBM_AA_EVEN) {
-      st_pl<2, SMALL>(Bb, plane, L::opp(i), 0, l.o0, C::st(v));
+      st_pl<2, ADDR>(Bb, plane, L::opp(i), 0, l.o0, C::st(v));
       continue;
     }
     if (i == 0) {
-      st_pl<0, SMALL>(Bb, plane, 0, 0, l.o0, C::st(v));
+      st_pl<0, ADDR>(Bb, plane, 0, 0, l.o0, C::st(v));
       continue;
     }
     // AA odd: push to x + c_i, or bounce back into the own slot opp(i)
@@ -49,14 +49,14 @@ __device__ __forceinline__ void store_node(const KArgs& a, const Lane& l, char* 
     if (r.wall) {
       float vb = v;
       if (cx != 0 && r.top) vb = v + (float)(-6.0 * L::w(i) * cx) * a.ulid;
-      st_pl<0, SMALL>(Bb, plane, L::opp(i), 0, l.o0, C::st(vb));
+      st_pl<0, ADDR>(Bb, plane, L::opp(i), 0, l.o0, C::st(vb));
       continue;
     }
     const uint32_t off = cx > 0 ? l.op : (cx < 0 ? l.om : l.o0);  // push target x + c_x
     if ((cx > 0 && l.wall_hi) || (cx < 0 && l.wall_lo))
-      st_pl<0, SMALL>(Bb, plane, L::opp(i), 0, l.o0, C::st(v));
+      st_pl<0, ADDR>(Bb, plane, L::opp(i), 0, l.o0, C::st(v));
     else
-      st_pl<0, SMALL>(Bb, plane, i, r.delta * (ptrdiff_t)TS, off, C::st(v));
+      st_pl<0, ADDR>(Bb, plane, i, r.delta * (ptrdiff_t)TS, off, C::st(v));
   }
 }
 
@@ -64,7 +64,7 @@ __device__ __forceinline__ void store_node(const KArgs& a, const Lane& l, char* 
 // k * kBlock + t for k < NPL, so every wave access stays one contiguous, aligned run of 64
 // elements; the loads of all NPL nodes are issued before the first collision, which
 // doubles the bytes a wave keeps in flight when NPL = 2.
-template <class L, int COLL, int ST, int PAT, int FM, bool SMALL, int NPL>
+template <class L, int COLL, int ST, int PAT, int FM, int ADDR, int NPL>
 __global__ void __launch_bounds__(kBlock)
 k_step(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T* B) {
   using C = Codec<ST>;
@@ -93,7 +93,7 @@ k_step(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T* B) 
   float F[NPL][3];
 #pragma unroll
   for (int k = 0; k < NPL; ++k) {
-    gather_issue<L, ST, PAT, SMALL>(a, l[k], Ab, plane, t[k], lid[k]);
+    gather_issue<L, ST, PAT, ADDR>(a, l[k], Ab, plane, t[k], lid[k]);
     F[k][0] = F[k][1] = F[k][2] = 0.0f;
     if (FM == kForceUniform) {
       F[k][0] = a.fx;
@@ -112,11 +112,11 @@ k_step(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T* B) 
 #pragma unroll
   for (int k = 0; k < NPL; ++k) {
     float h[Q];  // deviations f - w of the pulled populations
-    gather_finish<L, ST, PAT, SMALL>(l[k], Ab, plane, t[k], lid[k], h);
+    gather_finish<L, ST, PAT, ADDR>(l[k], Ab, plane, t[k], lid[k], h);
     float d[Q];
     collide_node<L, COLL, FM != kForceNone>(h, d, F[k], a.r0, a.r1, a.r2, a.r3);
     if (k > 0 && !act[k]) break;
-    store_node<L, ST, PAT, SMALL>(a, l[k], Bb, plane, d);
+    store_node<L, ST, PAT, ADDR>(a, l[k], Bb, plane, d);
   }
 }
 
@@ -146,45 +146,73 @@ inline KArgs with_npl(const KArgs& a, int npl) {
   return b;
 }
 
-template <class L, int COLL, int ST, int PAT, bool SMALL, int NPL>
+template <class L, int COLL, int ST, int PAT, int ADDR, int NPL>
 void launch_step_n(const KArgs& a0, int fmode, const void* src, void* dst, hipStream_t s) {
   using T = typename Codec<ST>::T;
   const KArgs a = with_npl(a0, NPL);
   const dim3 grid((unsigned)a.nblk), block(kBlock);
   if (fmode == kForceField)
-    k_step<L, COLL, ST, PAT, kForceField, SMALL, NPL><<<grid, block, 0, s>>>(a, (const T*)src,
+    k_step<L, COLL, ST, PAT, kForceField, ADDR, NPL><<<grid, block, 0, s>>>(a, (const T*)src,
                                                                              (T*)dst);
   else if (fmode == kForceUniform)
-    k_step<L, COLL, ST, PAT, kForceUniform, SMALL, NPL><<<grid, block, 0, s>>>(
+    k_step<L, COLL, ST, PAT, kForceUniform, ADDR, NPL><<<grid, block, 0, s>>>(
         a, (const T*)src, (T*)dst);
   else
-    k_step<L, COLL, ST, PAT, kForceNone, SMALL, NPL><<<grid, block, 0, s>>>(a, (const T*)src,
+    k_step<L, COLL, ST, PAT, kForceNone, ADDR, NPL><<<grid, block, 0, s>>>(a, (const T*)src,
                                                                             (T*)dst);
 }
 
 // Several nodes per lane only pay on rows at least one full segment long; shorter rows
 // would leave the extra node slots idle and keep their register cost.
-template <class L, int COLL, int ST, int PAT, bool SMALL>
+template <class L, int COLL, int ST, int PAT, int ADDR>
 void launch_step_f(const KArgs& a, int fmode, const void* src, void* dst, hipStream_t s) {
   constexpr int NPL = nodes_per_lane<L, ST, PAT>();
   if constexpr (NPL > 1) {
     if (a.nx >= kBlock * NPL) {
-      launch_step_n<L, COLL, ST, PAT, SMALL, NPL>(a, fmode, src, dst, s);
+      launch_step_n<L, COLL, ST, PAT, ADDR, NPL>(a, fmode, src, dst, s);
       return;
     }
   }
-  launch_step_n<L, COLL, ST, PAT, SMALL, 1>(a, fmode, src, dst, s);
+  launch_step_n<L, COLL, ST, PAT, ADDR, 1>(a, fmode, src, dst, s);
+}
+
+// Addressing mode of a state array of q planes (plane_ref in lbm_access.hpp): one
+// descriptor below 4 GiB; above, for the AA patterns, plane groups of 8 or 4 slots when
+// (G + 2) planes fit in 4 GiB (D3Q19 512^3: fp16 planes of 256 MiB -> 8, fp32 planes of
+// 512 MiB -> 4), else one descriptor per access.  Few descriptors keep the uniform
+// addressing in scalar registers: with one per plane and row, the two-nodes-per-lane
+// D3Q19 kernels spill scalar registers into vector lanes (~120 vector instructions per
+// node).  Same-box A/B at 512^3: groups make AA fp16 4.5% faster, but two-buffer fp16 2%
+// slower (and fp32 / bf16 within 0.6%), so the two-buffer kernels keep one descriptor per
+// access.
+inline int addr_mode(uint64_t q, uint64_t plane, bool grouped) {
+  const uint64_t lim = (uint64_t)1 << 32;
+  if (q * plane < lim) return 1;
+  if (!grouped) return 0;
+  if (10 * plane <= lim) return 8;
+  if (6 * plane <= lim) return 4;
+  return 0;
 }
 
 template <class L, int COLL, int ST, int PAT>
 int launch_step_t(const KArgs& a, int fmode, const void* src, void* dst, hipStream_t s) {
-  // one descriptor per array when the population array and the force field fit in 4 GiB
-  const uint64_t bytes = (uint64_t)L::Q * (uint64_t)a.N * sizeof(typename Codec<ST>::T);
+  const uint64_t plane = (uint64_t)a.N * sizeof(typename Codec<ST>::T);
   const uint64_t fbytes = (uint64_t)L::D * (uint64_t)a.N * sizeof(float);
-  if (bytes < ((uint64_t)1 << 32) && fbytes < ((uint64_t)1 << 32))
-    launch_step_f<L, COLL, ST, PAT, true>(a, fmode, src, dst, s);
+  const int mode = fbytes < ((uint64_t)1 << 32) ? addr_mode(L::Q, plane, PAT != PYLBM_AB) : 0;
+  if constexpr (PAT != PYLBM_AB) {
+    if (mode == 8) {
+      launch_step_f<L, COLL, ST, PAT, 8>(a, fmode, src, dst, s);
+      return (int)hipGetLastError();
+    }
+    if (mode == 4) {
+      launch_step_f<L, COLL, ST, PAT, 4>(a, fmode, src, dst, s);
+      return (int)hipGetLastError();
+    }
+  }
+  if (mode == 1)
+    launch_step_f<L, COLL, ST, PAT, 1>(a, fmode, src, dst, s);
   else
-    launch_step_f<L, COLL, ST, PAT, false>(a, fmode, src, dst, s);
+    launch_step_f<L, COLL, ST, PAT, 0>(a, fmode, src, dst, s);
   return (int)hipGetLastError();
 }
 

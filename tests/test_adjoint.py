@@ -92,6 +92,10 @@ VJP_CASES = {
     "d3q19-bgk-periodic": dict(shape=(12, 8, 6), tau=0.7, periodic=(True, True, True)),
     "d3q19-trt-walls": dict(shape=(10, 8, 6), tau=0.65, periodic=(True, False, False),
                             collision="trt", u_lid=0.03),
+    # rows of 600 nodes: two nodes per lane, a partial second segment, the x wrap on both
+    "d2q9-trt-wide-rows": dict(shape=(600, 5), tau=0.75, periodic=(True, False),
+                               collision="trt", u_lid=0.02),
+    "d2q9-bgk-wide-walls": dict(shape=(530, 4), tau=0.8, collision="bgk", u_lid=0.04),
 }
 
 

@@ -97,6 +97,11 @@ RANDOM_GEO = {
     "pery_inlet_mrt": dict(shape=(33, 17), periodic=(False, True), inlet=True, outlet=True,
                            collision="mrt"),
     "multi_segment_rows": dict(shape=(300, 9), inlet=True, outlet=True, collision="trt"),
+    # rows long enough for 2 (fp32) and 4 (16-bit) nodes per lane, with a partial segment
+    "wide_rows_lid": dict(shape=(1100, 7), u_lid=0.05, inlet=True, outlet=True,
+                          collision="mrt"),
+    "wide_rows_periodic_x": dict(shape=(1030, 6), periodic=(True, False), outlet=True,
+                                 force=(1e-5, 0.0), collision="bgk"),
     "d3q19_lid": dict(shape=(19, 11, 9), periodic=(True, False, False), u_lid=0.05,
                       collision="trt", force=(1e-5, 0.0, 0.0)),
     "d3q19_closed": dict(shape=(13, 12, 10), collision="bgk"),

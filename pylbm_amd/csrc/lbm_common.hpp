@@ -145,7 +145,8 @@ struct KArgs {
 
 // Workgroups are dispatched round-robin over the 8 XCDs.  With remap, XCD k processes the
 // k-th contiguous eighth of the row segments, so neighbouring segments (which share the
-// cache lines at their x edges) and neighbouring rows share one L2.
+// cache lines at their x edges) and neighbouring rows share one L2; without it the eight
+// XCDs sweep one common window of rows.  The host picks per lattice (make_kargs).
 __device__ __forceinline__ int remap_block(int b, int nblk, int remap) {
   if (!remap) return b;
   const int per_xcd = nblk >> 3;  // nblk % 8 == 0 when remap is set

@@ -141,7 +141,7 @@ inline KArgs with_npl(const KArgs& a, int npl) {
   b.rpw_log2 = 0;
   b.nseg = (a.nx + kBlock * npl - 1) / (kBlock * npl);
   b.nblk = b.nseg * a.ny * a.nz;
-  b.remap = (b.nblk % 8 == 0) ? 1 : 0;
+  b.remap = (a.remap && b.nblk % 8 == 0) ? 1 : 0;
   b.div_nseg = make_fastdiv((uint32_t)b.nseg);
   return b;
 }

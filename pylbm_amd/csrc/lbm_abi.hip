@@ -133,17 +133,12 @@ int make_kargs(const pylbm_params* p, KArgs& a, const char* what) {
   a.per = 0;
   for (int k = 0; k < dims; ++k)
     if (p->periodic[k]) a.per |= 1 << k;
-  // XCD-aware slab order (remap_block) for D3Q19 and for 2D states that fit in the
-  // 256 MiB Infinity Cache.  Large 2D states stream faster with consecutive workgroups
-  // round-robin over the 8 XCDs, all of them sweeping one window of rows together: 2-9%
-  // in same-box A/B at 4096^2 (every variant but bf16 BGK), with HBM traffic equal to the
-  // model either way.  A small state (the 1024 x 256 channel, 9 MiB) instead runs 30%
-  // slower without the slabs, whose rows then stay in each XCD's own L2.  In 3D the slabs
-  // measured faster for fp32 (1.6%) and bf16 (8%) and slower for fp16 (2-3%) at 512^3.
+  // workgroup order over the XCDs (remap_block, set_xcd_order)
   const int64_t q = p->lattice == PYLBM_D2Q9 ? 9 : 19;
   const int64_t item = (p->storage == PYLBM_F16 || p->storage == PYLBM_BF16) ? 2 : 4;
   const bool cached = 2 * q * (int64_t)p->n_nodes * item <= ((int64_t)256 << 20);
-  a.remap = ((p->lattice == PYLBM_D3Q19 || cached) && a.nblk % 8 == 0) ? 1 : 0;
+  set_xcd_order(a, cached ? kOrderSlabs
+                          : (p->lattice == PYLBM_D3Q19 ? kOrderChunk16 : kOrderNone));
   a.div_nseg = make_fastdiv((uint32_t)a.nseg);
   a.div_ny = make_fastdiv((uint32_t)p->ny);
   a.ulid = p->ulid;

@@ -149,7 +149,7 @@ k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f,
   constexpr int Q = L::Q;
   constexpr uint32_t TS = sizeof(T);
   Lane l;
-  const bool live = lane_setup<TS>(a, remap_block((int)blockIdx.x, a.nblk, a.remap), l);
+  const bool live = lane_setup<TS>(a, remap_block((int)blockIdx.x, a), l);
   float dsp = 0.f, dsm = 0.f;
   if (live) {
     const size_t N = (size_t)a.N;

@@ -131,8 +131,9 @@ struct KArgs {
   int sw_log2;   // log2 of the lanes one row segment takes (8, or 7 / 6 for short rows)
   int rpw_log2;  // log2 of the rows one workgroup serves (0, or 1 / 2 for short rows)
   int per;     // bit a: axis a periodic
-  int remap;   // 1: XCD-aware workgroup -> row-segment mapping
-  FastDiv div_nseg, div_ny;
+  int order;   // workgroup order over the XCDs (host policy, xcd_chunk)
+  int chunk;   // consecutive workgroups an XCD takes per turn (0: plain round-robin)
+  FastDiv div_nseg, div_ny, div_chunk;
   float ulid;
   float fx, fy, fz;
   float r0, r1, r2, r3;
@@ -143,14 +144,31 @@ struct KArgs {
   int outlet;          // 1: pressure outlet on the last column
 };
 
-// Workgroups are dispatched round-robin over the 8 XCDs.  With remap, XCD k processes the
-// k-th contiguous eighth of the row segments, so neighbouring segments (which share the
-// cache lines at their x edges) and neighbouring rows share one L2; without it the eight
-// XCDs sweep one common window of rows.  The host picks per lattice (make_kargs).
-__device__ __forceinline__ int remap_block(int b, int nblk, int remap) {
-  if (!remap) return b;
-  const int per_xcd = nblk >> 3;  // nblk % 8 == 0 when remap is set
-  return (b & 7) * per_xcd + (b >> 3);
+// Workgroups are dispatched round-robin over the 8 XCDs, so workgroup b runs on XCD b % 8.
+// With chunk C, XCD x takes C consecutive work items per turn: dispatch slot b maps to
+// item ((i / C) * 8 + x) * C + i % C, with x = b % 8 and i = b / 8.  C = nblk / 8 gives
+// every XCD its own slab (neighbouring segments and rows share one L2); small C keeps the
+// eight XCDs sweeping one window of rows together; C = 0 leaves the dispatch order.
+__device__ __forceinline__ int remap_block(int b, const KArgs& a) {
+  if (a.chunk == 0) return b;
+  const int x = b & 7, i = b >> 3;
+  const int turn = (int)a.div_chunk.div((uint32_t)i);
+  return (turn * 8 + x) * a.chunk + (i - turn * a.chunk);
+}
+
+// Host policy for the order (measured on MI355X, same-box A/B): states that fit in the
+// 256 MiB Infinity Cache -> slabs (the 1024 x 256 adjoint channel ran 30% slower without);
+// large D3Q19 states -> chunks of 16 (512^3 TRT: fp32 +2%, fp16 +6%, AA fp16 +4% over
+// slabs, bf16 -1%); large D2Q9 states -> plain round-robin (4096^2: 2-9% over slabs,
+// chunks of 16 or 64 in between).  HBM traffic equals the model in every order.
+enum XcdOrder { kOrderNone = 0, kOrderSlabs = 1, kOrderChunk16 = 2 };
+
+inline void set_xcd_order(KArgs& a, int order) {
+  a.order = order;
+  a.chunk = 0;
+  if (order == kOrderChunk16 && a.nblk % (8 * 16) == 0) a.chunk = 16;
+  else if (order != kOrderNone && a.nblk % 8 == 0) a.chunk = a.nblk / 8;
+  a.div_chunk = make_fastdiv((uint32_t)(a.chunk > 0 ? a.chunk : 1));
 }
 
 // ------------------------------------------------------------------------ collision

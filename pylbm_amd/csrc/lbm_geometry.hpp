@@ -122,7 +122,7 @@ k_step_geo(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T*
   constexpr uint32_t TS = sizeof(T);
 
   int y, z, seg;
-  if (!block_row(a, remap_block((int)blockIdx.x, a.nblk, a.remap), y, z, seg)) return;
+  if (!block_row(a, remap_block((int)blockIdx.x, a), y, z, seg)) return;
   Lane l[NPL];
   bool act[NPL];
 #pragma unroll

@@ -73,7 +73,7 @@ k_step(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T* B) 
   constexpr uint32_t TS = sizeof(T);
 
   int y, z, seg;
-  if (!block_row(a, remap_block((int)blockIdx.x, a.nblk, a.remap), y, z, seg)) return;
+  if (!block_row(a, remap_block((int)blockIdx.x, a), y, z, seg)) return;
   Lane l[NPL];
   bool act[NPL];
 #pragma unroll
@@ -141,7 +141,7 @@ inline KArgs with_npl(const KArgs& a, int npl) {
   b.rpw_log2 = 0;
   b.nseg = (a.nx + kBlock * npl - 1) / (kBlock * npl);
   b.nblk = b.nseg * a.ny * a.nz;
-  b.remap = (a.remap && b.nblk % 8 == 0) ? 1 : 0;
+  set_xcd_order(b, a.order);
   b.div_nseg = make_fastdiv((uint32_t)b.nseg);
   return b;
 }

@@ -177,18 +177,15 @@ void launch_step_f(const KArgs& a, int fmode, const void* src, void* dst, hipStr
 }
 
 // Addressing mode of a state array of q planes (plane_ref in lbm_access.hpp): one
-// descriptor below 4 GiB; above, for the AA patterns, plane groups of 8 or 4 slots when
-// (G + 2) planes fit in 4 GiB (D3Q19 512^3: fp16 planes of 256 MiB -> 8, fp32 planes of
-// 512 MiB -> 4), else one descriptor per access.  Few descriptors keep the uniform
-// addressing in scalar registers: with one per plane and row, the two-nodes-per-lane
-// D3Q19 kernels spill scalar registers into vector lanes (~120 vector instructions per
-// node).  Same-box A/B at 512^3: groups make AA fp16 4.5% faster, but two-buffer fp16 2%
-// slower (and fp32 / bf16 within 0.6%), so the two-buffer kernels keep one descriptor per
-// access.
-inline int addr_mode(uint64_t q, uint64_t plane, bool grouped) {
+// descriptor below 4 GiB; above, plane groups of 8 or 4 slots when (G + 2) planes fit in
+// 4 GiB (D3Q19 512^3: fp16 planes of 256 MiB -> 8, fp32 planes of 512 MiB -> 4), else one
+// descriptor per access.  Few descriptors keep the uniform addressing in scalar registers:
+// with one per plane and row, the two-nodes-per-lane D3Q19 kernels spill scalar registers
+// into vector lanes (~120 vector instructions per node).  Same-box A/B at 512^3 against
+// per-access descriptors: AA fp16 +4.5%, two-buffer fp16 +1.3%, fp32 +0.5%, bf16 -0.2%.
+inline int addr_mode(uint64_t q, uint64_t plane) {
   const uint64_t lim = (uint64_t)1 << 32;
   if (q * plane < lim) return 1;
-  if (!grouped) return 0;
   if (10 * plane <= lim) return 8;
   if (6 * plane <= lim) return 4;
   return 0;
@@ -198,21 +195,13 @@ template <class L, int COLL, int ST, int PAT>
 int launch_step_t(const KArgs& a, int fmode, const void* src, void* dst, hipStream_t s) {
   const uint64_t plane = (uint64_t)a.N * sizeof(typename Codec<ST>::T);
   const uint64_t fbytes = (uint64_t)L::D * (uint64_t)a.N * sizeof(float);
-  const int mode = fbytes < ((uint64_t)1 << 32) ? addr_mode(L::Q, plane, PAT != PYLBM_AB) : 0;
-  if constexpr (PAT != PYLBM_AB) {
-    if (mode == 8) {
-      launch_step_f<L, COLL, ST, PAT, 8>(a, fmode, src, dst, s);
-      return (int)hipGetLastError();
-    }
-    if (mode == 4) {
-      launch_step_f<L, COLL, ST, PAT, 4>(a, fmode, src, dst, s);
-      return (int)hipGetLastError();
-    }
+  const int mode = fbytes < ((uint64_t)1 << 32) ? addr_mode(L::Q, plane) : 0;
+  switch (mode) {
+    case 1: launch_step_f<L, COLL, ST, PAT, 1>(a, fmode, src, dst, s); break;
+    case 8: launch_step_f<L, COLL, ST, PAT, 8>(a, fmode, src, dst, s); break;
+    case 4: launch_step_f<L, COLL, ST, PAT, 4>(a, fmode, src, dst, s); break;
+    default: launch_step_f<L, COLL, ST, PAT, 0>(a, fmode, src, dst, s); break;
   }
-  if (mode == 1)
-    launch_step_f<L, COLL, ST, PAT, 1>(a, fmode, src, dst, s);
-  else
-    launch_step_f<L, COLL, ST, PAT, 0>(a, fmode, src, dst, s);
   return (int)hipGetLastError();
 }
 

@@ -120,15 +120,17 @@ k_step(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T* B) 
   }
 }
 
-// Nodes per lane, from A/B timing on MI355X (same box, alternating runs): serving more
-// nodes per lane keeps more bytes in flight per wave, which pays until register pressure
-// cuts occupancy.  D2Q9: 4 for 16-bit storage (+1-3% over 2), 2 for fp32 two-buffer (+4%
-// over 1), 1 for fp32 AA.  D3Q19 (19 populations per node): 2 for 16-bit (+22% on AA over
-// 1; 4 loses 15%), 1 for fp32 (2 loses 2%).
+// Nodes per lane, from A/B timing on MI355X (same box, alternating runs, with the
+// workgroup order of set_xcd_order): serving more nodes per lane keeps more bytes in
+// flight per wave, which pays until register pressure cuts occupancy or the longer row
+// segments spoil the DRAM locality of the common row window.  D2Q9: one node per lane
+// for the two-buffer kernels (4096^2 fp16 +5%, bf16 +4%, fp32 +3.6% over two; fp16 165k
+// vs 157k MLUPS), two for the 16-bit AA kernels (+1-2% over one), one for fp32 AA.
+// D3Q19 (19 populations per node): two for 16-bit, one for fp32 (two loses 3%).
 template <class L, int ST, int PAT>
 constexpr int nodes_per_lane() {
   constexpr bool half = sizeof(typename Codec<ST>::T) == 2;
-  if constexpr (L::Q == 9) return half ? 4 : (PAT == PYLBM_AB ? 2 : 1);
+  if constexpr (L::Q == 9) return (half && PAT != PYLBM_AB) ? 2 : 1;
   else return half ? 2 : 1;
 }
 

@@ -138,7 +138,7 @@ int make_kargs(const pylbm_params* p, KArgs& a, const char* what) {
   const int64_t item = (p->storage == PYLBM_F16 || p->storage == PYLBM_BF16) ? 2 : 4;
   const bool cached = 2 * q * (int64_t)p->n_nodes * item <= ((int64_t)256 << 20);
   set_xcd_order(a, cached ? kOrderSlabs
-                          : (p->lattice == PYLBM_D3Q19 ? kOrderChunk16 : kOrderNone));
+                          : (p->lattice == PYLBM_D3Q19 ? kOrderChunk16 : kOrderRun));
   a.div_nseg = make_fastdiv((uint32_t)a.nseg);
   a.div_ny = make_fastdiv((uint32_t)p->ny);
   a.ulid = p->ulid;

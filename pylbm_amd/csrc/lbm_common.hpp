@@ -159,14 +159,20 @@ __device__ __forceinline__ int remap_block(int b, const KArgs& a) {
 // Host policy for the order (measured on MI355X, same-box A/B): states that fit in the
 // 256 MiB Infinity Cache -> slabs (the 1024 x 256 adjoint channel ran 30% slower without);
 // large D3Q19 states -> chunks of 16 (512^3 TRT: fp32 +2%, fp16 +6%, AA fp16 +4% over
-// slabs, bf16 -1%); large D2Q9 states -> plain round-robin (4096^2: 2-9% over slabs,
-// chunks of 16 or 64 in between).  HBM traffic equals the model in every order.
-enum XcdOrder { kOrderNone = 0, kOrderSlabs = 1, kOrderChunk16 = 2 };
+// slabs, bf16 -1%); large D2Q9 states -> runs of 2 workgroups (16 at two nodes per lane):
+// 4096^2 BGK over plain round-robin fp16 +1.9%, bf16 +2.1%, AA fp32 +2.2%, fp32 +0.3%,
+// geometry fp32 -0.9%; round-robin itself beat slabs by 2-9%, and runs of 4 or 16
+// workgroups at one node per lane lose 1-10%.  The read traffic seen by the L2 grows by
+// the cache lines at workgroup edges that two XCDs both fetch: at 4096^2 fp32 reads are
+// 1.083x the model under round-robin, 1.042x under runs of 2, 1.0002x under slabs.
+// kOrderRun: runs of 2 workgroups per XCD turn at one node per lane, 16 at more.
+enum XcdOrder { kOrderNone = 0, kOrderSlabs = 1, kOrderChunk16 = 2, kOrderRun = 3 };
 
-inline void set_xcd_order(KArgs& a, int order) {
+inline void set_xcd_order(KArgs& a, int order, int npl = 1) {
   a.order = order;
   a.chunk = 0;
-  if (order == kOrderChunk16 && a.nblk % (8 * 16) == 0) a.chunk = 16;
+  const int run = order == kOrderChunk16 ? 16 : (order == kOrderRun ? (npl == 1 ? 2 : 16) : 0);
+  if (run > 0 && a.nblk % (8 * run) == 0) a.chunk = run;
   else if (order != kOrderNone && a.nblk % 8 == 0) a.chunk = a.nblk / 8;
   a.div_chunk = make_fastdiv((uint32_t)(a.chunk > 0 ? a.chunk : 1));
 }

@@ -193,9 +193,12 @@ k_links(const uint8_t* solid, LinkMask<L>* links, int nx, int ny, int nz, int N)
   links[n] = (LinkMask<L>)m;
 }
 
-// Nodes per lane of the geometry kernel: as the box kernel for D2Q9 (on rows at least one
-// segment long); one for D3Q19, where the bounce-back block's extra registers make two
-// nodes per lane 6% slower (sphere-array box, 256^3, fp16).
+// Nodes per lane of the geometry kernel (D2Q9 only on rows at least one segment long),
+// from A/B timing on the cylinder-array channel, 4096^2 TRT: one for fp32 (80.5k MLUPS vs
+// 76.8k at two), four for 16-bit storage (fp16 139.9k vs 138.3k at one, bf16 138.2k vs
+// 133.0k); the mask load and the bounce-back block shift the optimum away from the box
+// kernel's.  One for D3Q19, where the bounce-back block's extra registers make two nodes
+// per lane 6% slower (sphere-array box, 256^3, fp16).
 template <class L, int COLL, int ST, int NPL>
 int launch_geo_n(const KArgs& a0, int fmode, const void* src, void* dst, hipStream_t s) {
   using T = typename Codec<ST>::T;
@@ -212,7 +215,7 @@ int launch_geo_n(const KArgs& a0, int fmode, const void* src, void* dst, hipStre
 
 template <class L, int COLL, int ST>
 int launch_geo_t(const KArgs& a, int fmode, const void* src, void* dst, hipStream_t s) {
-  constexpr int NPL = L::Q == 9 ? nodes_per_lane<L, ST, PYLBM_AB>() : 1;
+  constexpr int NPL = L::Q == 9 ? (sizeof(typename Codec<ST>::T) == 2 ? 4 : 1) : 1;
   if constexpr (NPL > 1) {
     if (a.nx >= kBlock * NPL) return launch_geo_n<L, COLL, ST, NPL>(a, fmode, src, dst, s);
   }

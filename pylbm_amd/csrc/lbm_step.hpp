@@ -126,7 +126,8 @@ k_step(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T* B) 
 // segments spoil the DRAM locality of the common row window.  D2Q9: one node per lane
 // for the two-buffer kernels (4096^2 fp16 +5%, bf16 +4%, fp32 +3.6% over two; fp16 165k
 // vs 157k MLUPS), two for the 16-bit AA kernels (+1-2% over one), one for fp32 AA.
-// D3Q19 (19 populations per node): two for 16-bit, one for fp32 (two loses 3%).
+// D3Q19 (19 populations per node, 512^3 TRT): two for 16-bit (bf16 68.8k vs 66.0k MLUPS at
+// one, fp16 and AA-fp16 within 0.5%), one for fp32 (two loses 3%).
 template <class L, int ST, int PAT>
 constexpr int nodes_per_lane() {
   constexpr bool half = sizeof(typename Codec<ST>::T) == 2;
@@ -143,7 +144,7 @@ inline KArgs with_npl(const KArgs& a, int npl) {
   b.rpw_log2 = 0;
   b.nseg = (a.nx + kBlock * npl - 1) / (kBlock * npl);
   b.nblk = b.nseg * a.ny * a.nz;
-  set_xcd_order(b, a.order);
+  set_xcd_order(b, a.order, npl);
   b.div_nseg = make_fastdiv((uint32_t)b.nseg);
   return b;
 }

@@ -334,9 +334,11 @@ def main() -> int:
     variants = [head]
     if not a.quick:
         for coll in ("bgk", "trt", "mrt"):
-            for v in ("fused", "fused-aa", "fused-fp16", "fused-bf16", "fused-aa-fp16",
-                      "fused-aa-bf16"):
+            for v in ("fused", "fused-f32s", "fused-aa", "fused-fp16", "fused-bf16",
+                      "fused-aa-fp16", "fused-aa-bf16"):
                 if coll == "bgk" and v == "fused":
+                    continue
+                if coll != "bgk" and v == "fused-f32s":
                     continue
                 r = run_variant(v, problem_2d(coll, n, rho, u), (rho, u), a.steps, a.warmup)
                 variants.append(r)

@@ -25,6 +25,7 @@ _TABLE: dict[tuple[str, str], tuple[str, str]] = {
     ("hip", "fused-aa-fp16"): (_HIP, "HipFusedAAFP16"),
     ("hip", "fused-aa-bf16"): (_HIP, "HipFusedAABF16"),
     ("hip", "gather"): (_HIP, "HipGather"),
+    ("hip", "fused-f32s"): (_HIP, "HipFusedF32S"),
 }
 
 COLORS = {"hip": "#e4002b"}

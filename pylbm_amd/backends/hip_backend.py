@@ -274,6 +274,17 @@ class HipGather(HipFused):
     description = "hand-written HIP (gfx950), fused pull with obstacles / inlet / outlet, fp32"
 
 
+class HipFusedF32S(HipFused):
+    """The two-buffer kernel on fp32 storage of f - w_i: the same 4 bytes per population and
+    the same speed as ``fused``, with the stored values about w_i / |f - w_i| times finer,
+    so steady states driven by small forces keep their accuracy (no reference
+    counterpart; the adjoint rollouts store their states this way)."""
+
+    variant = "fused-f32s"
+    description = "hand-written HIP (gfx950), fused pull, two buffers, fp32 storage of f - w_i"
+    storage = "f32s"
+
+
 class HipFusedAA(HipFused):
     variant = "fused-aa"
     description = "hand-written HIP (gfx950), AA-pattern in-place streaming, fp32"

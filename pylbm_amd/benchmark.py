@@ -111,7 +111,8 @@ PLANS = {
     "full": dict(sizes_2d=[256, 512, 1024, 2048, 4096], sizes_3d=[64, 128, 256, 512],
                  repeats=5, min_time=0.5, copy_mib=4096),
 }
-VARIANTS = ["fused", "fused-aa", "fused-fp16", "fused-bf16", "fused-aa-fp16", "fused-aa-bf16"]
+VARIANTS = ["fused", "fused-f32s", "fused-aa", "fused-fp16", "fused-bf16", "fused-aa-fp16",
+            "fused-aa-bf16"]
 
 
 def bench_problem(dim: int, n: int, collision: str = "bgk"):
@@ -174,6 +175,7 @@ def measure(variant: str, dim: int, n: int, plan: dict, copy_bw: float,
            "median": med, "min": min(ml), "max": max(ml), "steps_per_repeat": steps,
            "bytes_per_unit": s.bytes_per_node, "effective_GBs": eff,
            "pct_copy_bw": 100.0 * eff / copy_bw, "pct_spec_bw": 100.0 * eff / SPEC_BANDWIDTH_GBS,
+           "flops_per_unit": None, "pct_peak_flops": None,
            "memory_MiB": s.memory_bytes() / 2**20, "notes": cls.description}
     del s
     torch.cuda.empty_cache()
@@ -184,6 +186,7 @@ def measure(variant: str, dim: int, n: int, plan: dict, copy_bw: float,
 
 def run(plan_name: str = "quick", out: str | None = None, log: Any = print) -> dict:
     """Run a benchmark plan over every hip variant; write JSON (pylbm/benchmark.py:242-324)."""
+    import datetime
     import json
     import platform
 
@@ -209,11 +212,13 @@ def run(plan_name: str = "quick", out: str | None = None, log: Any = print) -> d
         r = measure("fused", 2, n, plan, cbw, collision=coll, log=log)
         if r:
             runs.append(r)
-    result = {"machine": {"python": platform.python_version(), "torch": torch.__version__,
-                          "gpu": torch.cuda.get_device_name(0)},
+    date = datetime.datetime.now().astimezone().isoformat(timespec="seconds")
+    result = {"machine": {"python": platform.python_version(), "date": date,
+                          "torch": torch.__version__, "gpu": torch.cuda.get_device_name(0)},
               "ceiling": {"copy_bandwidth_GBs": cbw, "spec_bandwidth_GBs": SPEC_BANDWIDTH_GBS,
-                          **ceil},
-              "plan": plan_name, "wall_time_s": time.perf_counter() - t0, "runs": runs}
+                          "fp32_tflops": None, **ceil},
+              "plan": plan_name, "provisional": False,
+              "wall_time_s": time.perf_counter() - t0, "runs": runs}
     if out:
         with open(out, "w") as fh:
             json.dump(result, fh, indent=1)

@@ -1,8 +1,10 @@
-"""Command line interface: ``python -m pylbm_amd run | validate | bench | backends``.
+"""Command line interface: ``python -m pylbm_amd run | validate | bench | report | backends``.
 
 Mirrors the reference's ``pylbm`` console script (pylbm/cli.py:98-144) for the hip backend:
 ``run cavity --backend hip --variant fused|fused-aa|fused-fp16|fused-bf16`` runs the box
-kernels, ``run karman`` the obstacle / inlet / outlet kernel (variant ``gather``).  (The reference's ``kernels`` subcommand prints DSL source; there is none.)
+kernels, ``run karman`` the obstacle / inlet / outlet kernel (variant ``gather``).
+``validate`` runs the validation suite and ``report`` turns bench / validate records into
+Markdown tables.  (The reference's ``kernels`` subcommand prints DSL source; there is none.)
 """
 
 from __future__ import annotations
@@ -56,34 +58,37 @@ def _cmd_run(a: argparse.Namespace) -> None:
 
 
 def _cmd_validate(a: argparse.Namespace) -> None:
-    from .validation import ghia, poiseuille, shear_wave, taylor_green
+    from .validation.suite import run_suite
 
-    v = a.variant
-    r = shear_wave.run_shear_wave(0.8, n=32 if a.quick else 128, backend="hip", variant=v)
-    print(f"shear wave tau=0.8: nu rel err {r['rel_error']:.2e}")
-    r = taylor_green.run_taylor_green(n=16 if a.quick else 48, plane="xz", backend="hip",
-                                      variant=v)
-    print(f"TGV D3Q19: nu rel err {r['rel_error']:.2e}")
-    r = poiseuille.run_poiseuille(16 if a.quick else 32, collision="trt", backend="hip",
-                                  variant=v)
-    print(f"Poiseuille TRT: L2 {r['L2']:.2e}")
-    c = ghia.run_cavity(100, 64 if a.quick else 128, backend="hip", variant=v)
-    print(f"Ghia Re=100: converged={c.converged} Linf={c.errors['Linf']:.4f}")
-    if v.startswith("fused-aa"):
-        print("Karman: skipped (the AA variants take box geometries only)")
-        return
-    from .validation import karman
-
-    r = karman.run_karman(d=20 if a.quick else 40, backend="hip",
-                          variant="gather" if v == "fused" else v)
-    print(f"Karman D={r['d']}: St={r['St']:.4f} cd_max={r['cd_max']:.3f} "
-          f"cl_max={r['cl_max']:.3f} (reference intervals {karman.REFERENCE})")
+    run_suite(quick=a.quick, cases=a.cases, variant=a.variant, out=a.out,
+              vorticity_out=a.vorticity_out)
 
 
 def _cmd_bench(a: argparse.Namespace) -> None:
     from .benchmark import run
 
     run("full" if a.full else "dev" if a.dev else "quick", out=a.out)
+
+
+def _cmd_report(a: argparse.Namespace) -> None:
+    from pathlib import Path
+
+    from . import report
+
+    bench = report.load(a.bench) if a.bench else None
+    val = report.load(a.validation) if a.validation else None
+    if bench is None and val is None:
+        raise SystemExit("report: give --bench and/or --validation")
+    if a.readme:
+        path = Path(a.readme)
+        path.write_text(report.fill_blocks(path.read_text(), report.blocks_for(bench, val)))
+        print(f"updated {path}")
+    text = report.render(bench, val)
+    if a.out:
+        Path(a.out).write_text(text)
+        print(f"wrote {a.out}")
+    elif not a.readme:
+        print(text)
 
 
 def main(argv: list[str] | None = None) -> None:
@@ -103,9 +108,13 @@ def main(argv: list[str] | None = None) -> None:
     sp.add_argument("--steps", type=int, default=10000)
     sp.add_argument("--out", default=None, help="write rho, u (npz)")
     sp.set_defaults(func=_cmd_run)
-    sp = sub.add_parser("validate", help="analytic checks on the hip backend")
-    sp.add_argument("--quick", action="store_true")
-    sp.add_argument("--variant", default="fused")
+    sp = sub.add_parser("validate", help="run the validation suite on the hip backend")
+    sp.add_argument("--quick", action="store_true", help="small grids")
+    sp.add_argument("--cases", nargs="*", default=None,
+                    help="subset of: ghia shear poiseuille karman tgv")
+    sp.add_argument("--variant", default=None, help="run every case on this hip variant")
+    sp.add_argument("--out", default=None, help="write the record (JSON)")
+    sp.add_argument("--vorticity-out", default=None, help="write the Karman vorticity (npz)")
     sp.set_defaults(func=_cmd_validate)
     sp = sub.add_parser("bench", help="benchmark every hip variant (cavity, Re=1000)")
     g = sp.add_mutually_exclusive_group()
@@ -114,6 +123,13 @@ def main(argv: list[str] | None = None) -> None:
     g.add_argument("--full", action="store_true")
     sp.add_argument("--out", default=None)
     sp.set_defaults(func=_cmd_bench)
+    sp = sub.add_parser("report", help="Markdown tables from bench / validate JSON")
+    sp.add_argument("--bench", default=None, help="bench --out record or bench.py output")
+    sp.add_argument("--validation", default=None, help="validate --out record")
+    sp.add_argument("--out", default=None, help="write the report here (default: stdout)")
+    sp.add_argument("--readme", default=None,
+                    help="refresh the <!-- pylbm_amd:NAME --> blocks of this Markdown file")
+    sp.set_defaults(func=_cmd_report)
     a = ap.parse_args(argv)
     a.func(a)
 

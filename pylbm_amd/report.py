@@ -1,0 +1,269 @@
+"""Markdown report from benchmark and validation results.
+
+The report side of the reference's results pipeline (figures/make_figures.py:400-651,
+which fills README and paper tables from results/*.json): a headline, the performance
+ladder per lattice and collision, and the validation table.  It reads
+
+- a benchmark record in the reference's schema (``machine``, ``ceiling``, ``runs`` with
+  ``backend``, ``variant``, ``lattice``, ``collision``, ``size``, ``median``, ``min``,
+  ``max``, ``bytes_per_unit``, ``effective_GBs``, ``pct_copy_bw``, ``pct_spec_bw``), as
+  ``python -m pylbm_amd bench --out`` writes it, or the one-line JSON of ``bench.py``
+  (converted by ``from_bench_line``);
+- a validation record in the reference's schema, as ``python -m pylbm_amd validate --out``
+  writes it.
+
+The reference's own result files follow the same schema and render unchanged.  Blocks of a
+Markdown file between ``<!-- pylbm_amd:NAME -->`` and ``<!-- /pylbm_amd:NAME -->`` can be
+refreshed in place (``fill_blocks``).
+"""
+
+from __future__ import annotations
+
+import json
+import math
+import re
+from pathlib import Path
+from typing import Any, Iterable
+
+#: hip variants in ladder order: two-buffer before AA, fp32 before 16-bit storage
+HIP_ORDER = ["gather", "fused", "fused-f32s", "fused-aa", "fused-fp16", "fused-bf16",
+             "fused-aa-fp16", "fused-aa-bf16"]
+
+
+def load(path: str | Path) -> dict:
+    """A benchmark or validation record; a ``bench.py`` output file (its last non-empty
+    line is the JSON record) is converted to the benchmark schema."""
+    text = Path(path).read_text().strip()
+    try:
+        rec = json.loads(text)
+    except json.JSONDecodeError:
+        rec = json.loads(text.splitlines()[-1])
+    if "runs" not in rec and "variants" in rec and "metric" in rec:
+        rec = from_bench_line(rec)
+    return rec
+
+
+def from_bench_line(line: dict) -> dict:
+    """The benchmark schema from one ``bench.py`` JSON line (its ``variants`` block)."""
+    ceil = line.get("copy_ceiling", {})
+    spec = line.get("roofline", {}).get("peak", 8000.0)
+    copy_bw = ceil.get("copy_bandwidth_GBs") or ceil.get("hip_copy_GBs")
+    runs = []
+    for v in line.get("variants", []):
+        gbs = v["GBs"]
+        runs.append({
+            "backend": "hip", "variant": v["variant"], "lattice": v["lattice"],
+            "collision": v["collision"], "size": list(v["size"]),
+            "storage": v.get("storage"), "metric": "MLUPS", "value": v["MLUPS"],
+            "median": v["MLUPS"], "min": v["MLUPS"], "max": v["MLUPS"],
+            "repeats": 1, "steps_per_repeat": v.get("steps"),
+            "bytes_per_unit": v["bytes_per_update"], "effective_GBs": gbs,
+            "pct_copy_bw": 100.0 * gbs / copy_bw if copy_bw else None,
+            "pct_spec_bw": 100.0 * gbs / spec,
+            "workload": v.get("workload", "")})
+    return {"machine": {"gpu": line.get("device", "GPU")},
+            "ceiling": {"copy_bandwidth_GBs": copy_bw, "spec_bandwidth_GBs": spec},
+            "plan": "bench.py", "provisional": False, "runs": runs}
+
+
+# ----------------------------------------------------------------------------- selection
+def _nodes(run: dict) -> int:
+    return math.prod(int(s) for s in run["size"])
+
+
+def rung_key(run: dict) -> tuple[str, str]:
+    return run["backend"], run["variant"]
+
+
+def _rank(key: tuple[str, str]) -> tuple[int, int, str]:
+    b, v = key
+    if b == "hip" and v in HIP_ORDER:
+        return 0, HIP_ORDER.index(v), ""
+    return 1, 0, f"{b}/{v}"
+
+
+def ladder(bench: dict, lattice: str, collision: str = "bgk",
+           workload: bool = False) -> list[dict]:
+    """One run per (backend, variant): the largest size it measured (ties: the faster run).
+
+    Runs that carry a non-empty ``workload`` (obstacle channels and the like) are kept out
+    unless ``workload`` is True, in which case only they are taken."""
+    best: dict[tuple[str, str], dict] = {}
+    for r in bench.get("runs", []):
+        if r.get("lattice") != lattice or r.get("collision") != collision:
+            continue
+        if bool(r.get("workload")) != workload:
+            continue
+        k = rung_key(r)
+        cur = best.get(k)
+        if cur is None or (_nodes(r), r["median"]) > (_nodes(cur), cur["median"]):
+            best[k] = r
+    return [best[k] for k in sorted(best, key=_rank)]
+
+
+def ceiling_mlups(bench: dict, bytes_per_update: float) -> float | None:
+    """MLUPS at the measured copy bandwidth for a given traffic per node update."""
+    bw = bench.get("ceiling", {}).get("copy_bandwidth_GBs")
+    return bw * 1e9 / bytes_per_update / 1e6 if bw else None
+
+
+def _is16(run: dict) -> bool:
+    return "16" in run["variant"] or run.get("storage") in ("f16", "bf16")
+
+
+# ------------------------------------------------------------------------------- tables
+def _size(run: dict) -> str:
+    return "x".join(str(s) for s in run["size"])
+
+
+def _pct(x: float | None) -> str:
+    return "—" if x is None else f"{x:.0f}"
+
+
+def headline(bench: dict) -> list[str]:
+    """Best fp32 and 16-bit D2Q9 rates and the best fp32 D3Q19 rate, against the ceilings.
+    The reference's float64 NumPy rung is not a candidate."""
+    gpu = bench.get("machine", {}).get("gpu", "GPU")
+    bw = bench.get("ceiling", {}).get("copy_bandwidth_GBs")
+    spec = bench.get("ceiling", {}).get("spec_bandwidth_GBs")
+    out = []
+    d2 = [r for r in ladder(bench, "D2Q9") if r["backend"] != "numpy"]
+    f32 = [r for r in d2 if not _is16(r)]
+    f16 = [r for r in d2 if _is16(r)]
+    if f32:
+        b = max(f32, key=lambda r: r["median"])
+        out.append(f"D2Q9 fp32 on {gpu}: **{b['median']:,.0f} MLUPS** ({b['backend']}/"
+                   f"{b['variant']}, {_size(b)}) = {_pct(b.get('pct_copy_bw'))}% of the "
+                   f"measured {bw:,.0f} GB/s copy bandwidth, {_pct(b.get('pct_spec_bw'))}% "
+                   f"of the {spec:,.0f} GB/s spec.")
+    if f16:
+        b = max(f16, key=lambda r: r["median"])
+        line = (f"16-bit storage: **{b['median']:,.0f} MLUPS** ({b['backend']}/"
+                f"{b['variant']}, {_pct(b.get('pct_copy_bw'))}% of copy bandwidth at "
+                f"{b['bytes_per_unit']} B/update)")
+        if f32:
+            line += f", {b['median'] / max(r['median'] for r in f32):.2f}x the best fp32 rung"
+        out.append(line + ".")
+    d3 = [r for r in ladder(bench, "D3Q19") + ladder(bench, "D3Q19", "trt")
+          if r["backend"] != "numpy" and not _is16(r)]
+    if d3:
+        b = max(d3, key=lambda r: r["median"])
+        out.append(f"D3Q19 fp32: **{b['median']:,.0f} MLUPS** ({b['backend']}/{b['variant']}, "
+                   f"{b['collision'].upper()}, {_size(b)}, {_pct(b.get('pct_copy_bw'))}% of "
+                   f"copy bandwidth).")
+    return out
+
+
+def performance_table(bench: dict, collisions: Iterable[str] = ("bgk", "trt", "mrt")) -> str:
+    """Markdown ladder: one row per lattice, collision and rung, plus obstacle workloads."""
+    rows = ["| lattice | collision | rung | size | MLUPS | B/update | GB/s | % copy | % spec |",
+            "|---|---|---|---|---:|---:|---:|---:|---:|"]
+    for lat in ("D2Q9", "D3Q19"):
+        for coll in collisions:
+            for wl in (False, True):
+                for r in ladder(bench, lat, coll, workload=wl):
+                    name = f"{r['backend']}/{r['variant']}"
+                    if wl:
+                        name += f" ({r['workload']})"
+                    rows.append(f"| {lat} | {coll} | {name} | {_size(r)} | "
+                                f"{r['median']:,.0f} | {r['bytes_per_unit']} | "
+                                f"{r['effective_GBs']:,.0f} | {_pct(r.get('pct_copy_bw'))} | "
+                                f"{_pct(r.get('pct_spec_bw'))} |")
+    bw = bench.get("ceiling", {}).get("copy_bandwidth_GBs")
+    note = (f"Ceiling: measured device copy bandwidth {bw:,.0f} GB/s"
+            f" = {ceiling_mlups(bench, 72):,.0f} / {ceiling_mlups(bench, 36):,.0f} MLUPS for "
+            f"D2Q9 fp32 / 16-bit and {ceiling_mlups(bench, 152):,.0f} / "
+            f"{ceiling_mlups(bench, 76):,.0f} for D3Q19." if bw else "")
+    return "\n".join(rows) + ("\n\n" + note if note else "")
+
+
+def validation_table(val: dict) -> str:
+    """Markdown table of every validation block present in the record."""
+    rows = ["| check | reference | result |", "|---|---|---|"]
+    for r in val.get("ghia", []):
+        e = r["errors"]
+        rows.append(f"| cavity Re={r['re']}, {r['n']}², {r['collision']}, {r['backend']} | "
+                    f"Ghia et al. 1982 centrelines | L2 = {e['L2']:.4f}, Linf = "
+                    f"{e['Linf']:.4f} (units of U), steps {r['steps']}, converged "
+                    f"{r['converged']} |")
+    sw = val.get("shear_wave", [])
+    if sw:
+        backends = ", ".join(sorted({r["backend"] for r in sw}))
+        rows.append(f"| shear-wave decay, tau {min(r['tau'] for r in sw)}–"
+                    f"{max(r['tau'] for r in sw)}, n = {sw[0]['n']} ({backends}) | "
+                    f"nu = (tau − 1/2)/3 | max rel. error "
+                    f"{100 * max(r['rel_error'] for r in sw):.2f}% |")
+    for coll, c in val.get("poiseuille", {}).items():
+        if "runs" in c:
+            hs = ", ".join(str(r["h"]) for r in c["runs"])
+            order = "—" if c.get("order") is None else f"{c['order']:.3f}"
+            rows.append(f"| Poiseuille {coll.upper()}, H = {hs} ({c['runs'][0]['backend']}) | "
+                        f"analytic parabola | order {order}, max L2 "
+                        f"{max(r['L2'] for r in c['runs']):.1e} |")
+        else:
+            rows.append(f"| Poiseuille {coll}, H = {c['h']} ({c['backend']}) | analytic "
+                        f"parabola | L2 {c['L2']:.1e}, Linf {c['Linf']:.1e} |")
+    k = val.get("karman")
+    if k:
+        ref = k.get("reference", {})
+        span = ", ".join(f"{name} {lo}–{hi}" for name, (lo, hi) in ref.items())
+        rows.append(f"| cylinder 2D-2, D = {k['d']} ({k['backend']}) | Schäfer–Turek 1996"
+                    f"{': ' + span if span else ''} | St = {k['St']:.4f}, cD,max = "
+                    f"{k['cd_max']:.3f}, cL,max = {k['cl_max']:.3f} |")
+    tg = val.get("taylor_green_3d", [])
+    for b in sorted({r["backend"] for r in tg}):
+        sel = [r for r in tg if r["backend"] == b]
+        planes = "/".join(r["plane"] for r in sel)
+        rows.append(f"| D3Q19 Taylor–Green, {sel[0]['n']}³, {planes} ({b}) | "
+                    f"nu = (tau − 1/2)/3 | max rel. error "
+                    f"{100 * max(r['rel_error'] for r in sel):.2f}% |")
+    eq = val.get("equivalence", [])
+    for st in sorted({r.get("storage", "f32") for r in eq}):
+        sel = [r for r in eq if r.get("storage", "f32") == st]
+        rows.append(f"| equivalence, {st} storage, {len(sel)} runs | float64 solver | "
+                    f"max abs Δf {max(r['max_abs_diff'] for r in sel):.2e} |")
+    return "\n".join(rows)
+
+
+def render(bench: dict | None = None, val: dict | None = None) -> str:
+    """The whole report as Markdown."""
+    parts = []
+    if bench is not None:
+        parts += ["## Headline", "", *[f"- {h}" for h in headline(bench)], "",
+                  "## Performance", "", performance_table(bench), ""]
+    if val is not None:
+        parts += ["## Validation", "", validation_table(val), ""]
+    return "\n".join(parts)
+
+
+_BLOCK = re.compile(r"(<!-- pylbm_amd:(?P<name>[\w-]+) -->\n).*?(\n?<!-- /pylbm_amd:(?P=name) -->)",
+                    re.S)
+
+
+def fill_blocks(text: str, blocks: dict[str, str]) -> str:
+    """Replace the body of every ``<!-- pylbm_amd:NAME -->`` block named in ``blocks``;
+    a name without its block in ``text`` raises KeyError."""
+    seen = set()
+
+    def sub(m: re.Match) -> str:
+        name = m.group("name")
+        if name not in blocks:
+            return m.group(0)
+        seen.add(name)
+        return m.group(1) + blocks[name].strip() + "\n" + m.group(3).lstrip("\n")
+
+    out = _BLOCK.sub(sub, text)
+    missing = set(blocks) - seen
+    if missing:
+        raise KeyError(f"no block for {sorted(missing)}")
+    return out
+
+
+def blocks_for(bench: dict | None, val: dict | None) -> dict[str, str]:
+    out = {}
+    if bench is not None:
+        out["headline"] = "\n".join(f"- {h}" for h in headline(bench))
+        out["performance"] = performance_table(bench)
+    if val is not None:
+        out["validation"] = validation_table(val)
+    return out

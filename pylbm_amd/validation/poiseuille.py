@@ -26,7 +26,8 @@ def analytic(h: int, force: float, nu: float) -> np.ndarray:
 def run_poiseuille(h: int, tau: float = 0.8, u_max: float = 0.02, collision: str = "bgk",
                    nx: int = 4, backend: str | None = None, variant: str | None = None,
                    factory: Any = None, tol: float = 1e-9, max_steps: int = 2_000_000) -> dict:
-    """Step to steady state; relative L2 / Linf errors of the profile against the parabola.
+    """Step to steady state; relative L2 / Linf errors of the profile against the parabola,
+    and the profile itself (y / h at the nodes, u / u_max).
 
     The check interval is half a viscous time h^2 / (2 nu); the run stops when the largest
     profile change per check falls below ``tol`` u_max, or at the round-off floor (no
@@ -54,7 +55,8 @@ def run_poiseuille(h: int, tau: float = 0.8, u_max: float = 0.02, collision: str
     return {"h": h, "tau": tau, "collision": collision, "backend": solver_name(s),
             "steps": s.steps_done,
             "L2": float(np.linalg.norm(err) / np.linalg.norm(ref)),
-            "Linf": float(np.abs(err).max() / ref.max())}
+            "Linf": float(np.abs(err).max() / ref.max()),
+            "y": ((np.arange(h) + 0.5) / h).tolist(), "u": (prof / u_max).tolist()}
 
 
 def convergence(hs: tuple[int, ...] = (8, 16, 32, 64), **kw: Any) -> dict:

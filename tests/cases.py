@@ -27,7 +27,7 @@ EQUIVALENCE = {
 }
 
 #: reference tolerances (tests/test_equivalence.py:21)
-TOL = {"f32": 3e-6, "f16": 1e-4, "bf16": 6e-4}
+TOL = {"f32": 3e-6, "f32s": 3e-6, "f16": 1e-4, "bf16": 6e-4}
 
 
 def golden(name: str) -> dict:

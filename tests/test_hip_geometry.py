@@ -24,7 +24,7 @@ from tests.cases import GEO_TRAJ, TOL, golden, karman6_kwargs
 
 pytestmark = pytest.mark.gpu
 
-AB_VARIANTS = ["gather", "fused", "fused-fp16", "fused-bf16"]
+AB_VARIANTS = ["gather", "fused", "fused-f32s", "fused-fp16", "fused-bf16"]
 
 #: fp32 bound: the reference's (tests/test_equivalence.py:21).
 TOL_F32 = 3e-6
@@ -50,7 +50,9 @@ def rounding_bound(case: R.Case, steps: int, storage: str, f_ref: np.ndarray) ->
 
 
 def bound(case: R.Case, steps: int, storage: str, f_ref: np.ndarray) -> float:
-    return TOL_F32 if storage == "f32" else rounding_bound(case, steps, storage, f_ref)
+    if storage in ("f32", "f32s"):
+        return TOL_F32
+    return rounding_bound(case, steps, storage, f_ref)
 
 
 @pytest.fixture(scope="module")
@@ -125,7 +127,8 @@ def test_random_geometry_matches_oracle(name):
         s = _make(variant, p)
         s.step(40)
         err = np.abs(s.populations() - ref).max()
-        tol = 1e-6 if s.storage == "f32" else rounding_bound(case, 40, s.storage, ref)
+        tol = 1e-6 if s.storage in ("f32", "f32s") else rounding_bound(case, 40, s.storage,
+                                                                        ref)
         print(f"{name} hip/{variant}: max|df| = {err:.3e} (bound {tol:.2e})")
         assert err < tol
 

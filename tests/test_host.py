@@ -118,7 +118,7 @@ def test_initial_populations_match_oracle():
 
 def test_registry_never_crashes():
     st = B.status()
-    assert len(st) == len(B.entries()) == 7
+    assert len(st) == len(B.entries()) == 8
     for b, v, ok, why in st:
         assert b == "hip"
         assert ok or why  # unavailable entries say why
@@ -132,7 +132,7 @@ def test_unknown_entry_raises_keyerror():
 def test_supports_geometry_on_two_buffer_variants_only():
     solid = np.zeros((8, 8), dtype=bool)
     geo = Problem((8, 8), solid=solid, inlet=np.zeros(8), outlet=True)
-    for v in ("fused", "fused-fp16", "fused-bf16", "gather"):
+    for v in ("fused", "fused-f32s", "fused-fp16", "fused-bf16", "gather"):
         assert B.get("hip", v).supports(geo)[0], v
     for v in ("fused-aa", "fused-aa-fp16", "fused-aa-bf16"):
         ok, why = B.get("hip", v).supports(geo)

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PYLBM_HIP_ABI_VERSION 2
+#define PYLBM_HIP_ABI_VERSION 3
 
 enum { PYLBM_D2Q9 = 0, PYLBM_D3Q19 = 1 };
 enum { PYLBM_BGK = 0, PYLBM_TRT = 1, PYLBM_MRT = 2 };
@@ -73,7 +73,10 @@ typedef struct pylbm_params {
   const float* inlet;   /* NULL, or u_x(y) of the velocity inlet on the x = 0 wall (ny
                            fp32 values; D2Q9; no effect when x is periodic) */
   int32_t outlet;       /* 1: rho = 1 pressure outlet on the x = nx - 1 column (D2Q9) */
-  int32_t reserved;     /* must be 0 */
+  int32_t lam_storage;  /* element type of the adjoint states lam of the pylbm_hip_adjoint_*
+                           entries: PYLBM_F32 (0) or PYLBM_F16 (values rounded to nearest
+                           fp16 on store, all arithmetic fp32); the forward entries ignore it.
+                           No reference counterpart. */
 } pylbm_params;
 
 /* ABI version (PYLBM_HIP_ABI_VERSION of the library). */
@@ -181,7 +184,8 @@ int pylbm_hip_copy(const void* src, void* dst, int64_t nbytes, void* stream);
  * torch.no_grad(), pylbm/backends/torch_backend.py:74,128,152,175,257).  Used by the
  * checkpointed rollout behind torch.autograd.Function (pylbm_amd/adjoint.py):
  *   seed -> for t = T-1 .. 0: adjoint_step(f^t) -> init.
- * Adjoint states lam are fp32 (q, N); dF is fp32 (d, N), accumulated in place.
+ * Adjoint states lam are (q, N) arrays of p->lam_storage elements (fp32 or fp16); dF is fp32
+ * (d, N), accumulated in place.
  */
 
 /* Doubles of the per-workgroup dL/ds+- partial-sum buffer pylbm_hip_adjoint_step needs
@@ -192,28 +196,28 @@ int64_t pylbm_hip_adjoint_workspace(const pylbm_params* p);
  * u = (sum c f - F/2)/rho the post-collision velocity: writes lam = dL/df_T and, with a
  * force field, initialises dF = dL/dF of that velocity definition (-grad_u / (2 rho)). */
 int pylbm_hip_adjoint_seed(const pylbm_params* p, const void* f_T, const float* grad_u,
-                           float* lam, float* dF, void* stream);
+                           void* lam, float* dF, void* stream);
 
 /* One adjoint step: lam_out = (d f^{t+1} / d f^t)^T lam_in, with f_t the stored state the
  * forward step read.  Accumulates dF += dL/dF(x) (force field only) and adds the step's
  * dL/ds+ and dL/ds- (BGK: both into slot 0 pattern [s+, 0]) to dtau_part[2 g], [2 g + 1]
  * for workgroup g; the caller zeroes dtau_part once and sums it at the end. */
-int pylbm_hip_adjoint_step(const pylbm_params* p, const void* f_t, const float* lam_in,
-                           float* lam_out, float* dF, double* dtau_part, void* stream);
+int pylbm_hip_adjoint_step(const pylbm_params* p, const void* f_t, const void* lam_in,
+                           void* lam_out, float* dF, double* dtau_part, void* stream);
 
 /* nsteps adjoint steps over a stored segment, in reverse: for k = nsteps-1 .. 0,
  * lam_out = adjoint of step(states[k]) applied to lam_in, ping-ponging lam_a (input on
  * entry) and lam_b; *which = 0 if the final adjoint state is in lam_a, 1 if in lam_b.
  * dF and dtau_part accumulate as in pylbm_hip_adjoint_step. */
 int pylbm_hip_adjoint_run(const pylbm_params* p, const void* const* states, int64_t nsteps,
-                          float* lam_a, float* lam_b, float* dF, double* dtau_part, int* which,
+                          void* lam_a, void* lam_b, float* dF, double* dtau_part, int* which,
                           void* stream);
 
 /* Gradient of the initial equilibrium f0 = feq(rho0, u0) (float64 fields; NULL rho0 = 1,
  * NULL u0 = 0): du0 (d, N) and drho0 (N) in float64 from lam0 = dL/df0 (either output may
  * be NULL). */
 int pylbm_hip_adjoint_init(const pylbm_params* p, const double* rho0, const double* u0,
-                           const float* lam0, double* du0, double* drho0, void* stream);
+                           const void* lam0, double* du0, double* drho0, void* stream);
 
 #ifdef __cplusplus
 }

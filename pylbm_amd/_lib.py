@@ -12,7 +12,7 @@ from pathlib import Path
 from typing import Any
 
 LIB_PATH = Path(__file__).resolve().parent / "libpylbm_hip.so"
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 D2Q9, D3Q19 = 0, 1
 BGK, TRT, MRT = 0, 1, 2
@@ -56,7 +56,7 @@ class Params(ctypes.Structure):
         ("links", ctypes.c_void_p),
         ("inlet", ctypes.c_void_p),
         ("outlet", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("lam_storage", ctypes.c_int32),
     ]
 
 

@@ -11,7 +11,10 @@ torch loss of u back-propagates to
 
 Forward states are checkpointed in HBM every ``checkpoint`` steps; the backward pass
 recomputes one segment at a time (all its states kept in HBM), then runs the adjoint steps
-of that segment in reverse with ``pylbm_hip_adjoint_step``.  The last segment's states are
+of that segment in reverse with ``pylbm_hip_adjoint_step``.  The adjoint states are fp32 by
+default; ``lam_storage="f16"`` stores them as fp16 (all adjoint arithmetic stays fp32), which
+cuts an adjoint step's traffic from q*(s + 8) to q*(s + 4) bytes per node at a gradient error
+of a few 1e-4 relative (measured against float64 in the tests).  The last segment's states are
 kept from the forward pass, so it is not recomputed.  The reference has no gradient path;
 the gradients are checked against float64 torch autograd of a restated step and against
 central finite differences.
@@ -20,6 +23,7 @@ central finite differences.
 from __future__ import annotations
 
 import ctypes
+import math
 from dataclasses import dataclass
 from typing import Any
 
@@ -30,6 +34,20 @@ from .problem import Problem
 
 _TORCH_DTYPES = {"f32": torch.float32, "f32s": torch.float32, "f16": torch.float16,
                  "bf16": torch.bfloat16}
+_LAM_DTYPES = {"f32": torch.float32, "f16": torch.float16}
+
+#: fp16 adjoint states: the terminal gradient is scaled by a power of two so that its
+#: largest component lands near this value, well inside fp16's normal range
+#: [6.1e-5, 65504] for the states it seeds (|lam| <= 2 |dL/du| / rho at the seed, and the
+#: adjoint of the diffusive step does not grow it); every gradient is divided by the scale.
+_LAM16_TARGET = 2.0 ** 8
+
+
+def lam16_scale(gmax: float) -> float:
+    """Power-of-two scale for fp16 adjoint states seeded by a gradient of max-abs gmax."""
+    if not (gmax > 0.0) or gmax != gmax or gmax == float("inf"):
+        return 1.0
+    return 2.0 ** round(math.log2(_LAM16_TARGET / gmax))
 
 
 def _p(t: Any) -> int | None:
@@ -60,9 +78,13 @@ class RolloutConfig:
     #: "f32s" (default) stores f - w in fp32: the same 4 bytes as "f32", but the
     #: non-equilibrium part that tau acts on keeps its digits; "f16"/"bf16" halve the bytes
     storage: str = "f32s"
+    #: element type of the adjoint states between steps: "f32" or "f16" (fp32 arithmetic)
+    lam_storage: str = "f32"
 
     def __post_init__(self) -> None:
         p = self.problem
+        if self.lam_storage not in _LAM_DTYPES:
+            raise ValueError(f"lam_storage must be one of {sorted(_LAM_DTYPES)}")
         if p.collision not in ("bgk", "trt"):
             raise ValueError("the adjoint covers BGK and TRT")
         if not p.is_box():
@@ -104,6 +126,7 @@ class _Engine:
         for k, r in enumerate(rates):
             prm.rates[k] = r
         prm.force_field = _p(force_field)
+        prm.lam_storage = _lib.STORAGE_IDS[cfg.lam_storage]
         self.prm = prm
         self.ref = ctypes.byref(prm)
 
@@ -175,7 +198,12 @@ class _RolloutFn(torch.autograd.Function):
         K = (T + C - 1) // C
         q, d, N = eng.q, eng.d, eng.N
         G = grad_u.detach().to(torch.float32).contiguous()
-        lam = [torch.empty(q * N, dtype=torch.float32, device=eng.dev) for _ in range(2)]
+        scale = 1.0
+        if cfg.lam_storage == "f16":
+            scale = lam16_scale(float(G.abs().max()))
+            G = G * scale
+        ldt = _LAM_DTYPES[cfg.lam_storage]
+        lam = [torch.empty(q * N, dtype=ldt, device=eng.dev) for _ in range(2)]
         dF = torch.zeros(d * N, dtype=torch.float32, device=eng.dev) if eng.ff is not None \
             else None
         nw = int(_lib.load().pylbm_hip_adjoint_workspace(eng.ref))
@@ -210,13 +238,16 @@ class _RolloutFn(torch.autograd.Function):
                 if need_rho else None
             _lib.call("pylbm_hip_adjoint_init", eng.ref, _p(ctx.r0d), _p(ctx.u0d), _p(lam0),
                       _p(du), _p(dr), _s())
-            du0, drho0 = du, dr
+            du0 = du if du is None or scale == 1.0 else du / scale
+            drho0 = dr if dr is None or scale == 1.0 else dr / scale
         dtau = None
         if need_tau:
             sums = part.view(-1, 2).sum(0)
             a, b = tau_rate_derivatives(eng.p, eng.tau)
-            dtau = (sums[0] * a + sums[1] * b).reshape(())
+            dtau = ((sums[0] * a + sums[1] * b) / scale).reshape(())
         dFo = dF.view((d,) + eng.p.array_shape) if (dF is not None and need_F) else None
+        if dFo is not None and scale != 1.0:
+            dFo = dFo / scale
         ctx.eng = ctx.ckpts = ctx.fT = None
         return dFo, dtau, du0, drho0, None
 

@@ -88,7 +88,8 @@ int check_params(const pylbm_params* p, const char* what) {
     return fail(PYLBM_EINVAL, "%s: a moving lid needs a wall on the last axis", what);
   if (p->lattice == PYLBM_D2Q9 && p->force_field == nullptr && p->force[2] != 0.0f)
     return fail(PYLBM_EINVAL, "%s: D2Q9 has no z force", what);
-  if (p->reserved != 0) return fail(PYLBM_EINVAL, "%s: reserved must be 0", what);
+  if (p->lam_storage != PYLBM_F32 && p->lam_storage != PYLBM_F16)
+    return fail(PYLBM_EINVAL, "%s: lam_storage must be PYLBM_F32 or PYLBM_F16", what);
   if (p->outlet != 0 && p->outlet != 1) return fail(PYLBM_EINVAL, "%s: outlet must be 0 or 1", what);
   if (p->links || p->inlet || p->outlet) {
     if ((p->inlet || p->outlet) && p->lattice != PYLBM_D2Q9)
@@ -357,20 +358,21 @@ static int check_adjoint(const pylbm_params* p, const char* what) {
 }
 
 int pylbm_hip_adjoint_seed(const pylbm_params* p, const void* f_T, const float* grad_u,
-                           float* lam, float* dF, void* stream) {
+                           void* lam, float* dF, void* stream) {
   if (int e = check_adjoint(p, "pylbm_hip_adjoint_seed")) return e;
   if (!f_T || !grad_u || !lam) return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_seed: NULL buffer");
   if (p->force_field && !dF)
     return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_seed: a force field needs dF");
   KArgs a;
   if (int e = make_kargs(p, a, "pylbm_hip_adjoint_seed")) return e;
-  return hip_fail(launch_adjoint_seed(p->lattice, a, p->storage, p->force_field != nullptr,
-                                      f_T, grad_u, lam, dF, (hipStream_t)stream),
+  return hip_fail(launch_adjoint_seed(p->lattice, a, p->storage, p->lam_storage,
+                                      p->force_field != nullptr, f_T, grad_u, lam, dF,
+                                      (hipStream_t)stream),
                   "pylbm_hip_adjoint_seed");
 }
 
-int pylbm_hip_adjoint_step(const pylbm_params* p, const void* f_t, const float* lam_in,
-                           float* lam_out, float* dF, double* dtau_part, void* stream) {
+int pylbm_hip_adjoint_step(const pylbm_params* p, const void* f_t, const void* lam_in,
+                           void* lam_out, float* dF, double* dtau_part, void* stream) {
   if (int e = check_adjoint(p, "pylbm_hip_adjoint_step")) return e;
   if (!f_t || !lam_in || !lam_out || !dtau_part)
     return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_step: NULL buffer");
@@ -380,14 +382,14 @@ int pylbm_hip_adjoint_step(const pylbm_params* p, const void* f_t, const float* 
     return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_step: a force field needs dF");
   KArgs a;
   if (int e = make_kargs(p, a, "pylbm_hip_adjoint_step")) return e;
-  return hip_fail(launch_adjoint_step(p->lattice, a, p->collision, p->storage,
+  return hip_fail(launch_adjoint_step(p->lattice, a, p->collision, p->storage, p->lam_storage,
                                       p->force_field != nullptr, f_t, lam_in, lam_out, dF,
                                       dtau_part, (hipStream_t)stream),
                   "pylbm_hip_adjoint_step");
 }
 
 int pylbm_hip_adjoint_run(const pylbm_params* p, const void* const* states, int64_t nsteps,
-                          float* lam_a, float* lam_b, float* dF, double* dtau_part, int* which,
+                          void* lam_a, void* lam_b, float* dF, double* dtau_part, int* which,
                           void* stream) {
   if (int e = check_adjoint(p, "pylbm_hip_adjoint_run")) return e;
   if (nsteps < 0 || !which) return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_run: bad arguments");
@@ -397,16 +399,16 @@ int pylbm_hip_adjoint_run(const pylbm_params* p, const void* const* states, int6
     return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_run: a force field needs dF");
   KArgs a;
   if (int e = make_kargs(p, a, "pylbm_hip_adjoint_run")) return e;
-  float* lin = lam_a;
-  float* lout = lam_b;
+  void* lin = lam_a;
+  void* lout = lam_b;
   for (int64_t k = nsteps - 1; k >= 0; --k) {
     if (!states[k]) return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_run: states[%lld] is NULL",
                                 (long long)k);
-    if (int e = launch_adjoint_step(p->lattice, a, p->collision, p->storage,
+    if (int e = launch_adjoint_step(p->lattice, a, p->collision, p->storage, p->lam_storage,
                                     p->force_field != nullptr, states[k], lin, lout, dF,
                                     dtau_part, (hipStream_t)stream))
       return hip_fail(e, "pylbm_hip_adjoint_run");
-    float* t = lin;
+    void* t = lin;
     lin = lout;
     lout = t;
   }
@@ -415,11 +417,12 @@ int pylbm_hip_adjoint_run(const pylbm_params* p, const void* const* states, int6
 }
 
 int pylbm_hip_adjoint_init(const pylbm_params* p, const double* rho0, const double* u0,
-                           const float* lam0, double* du0, double* drho0, void* stream) {
+                           const void* lam0, double* du0, double* drho0, void* stream) {
   if (int e = check_params(p, "pylbm_hip_adjoint_init")) return e;
   if (!lam0) return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_init: NULL lam0");
   if (!du0 && !drho0) return 0;
-  return hip_fail(launch_adjoint_init(p->lattice, (int)p->n_nodes, rho0, u0, lam0, du0, drho0,
+  return hip_fail(launch_adjoint_init(p->lattice, p->lam_storage, (int)p->n_nodes, rho0, u0,
+                                      lam0, du0, drho0,
                                       (hipStream_t)stream),
                   "pylbm_hip_adjoint_init");
 }

@@ -140,14 +140,17 @@ __device__ __forceinline__ float wave_sum_f(float v) {
   return v;
 }
 
-template <class L, int COLL, int ST, int FM, int ADDR>
+// LT is the element type of the adjoint states lam_in / lam_out: float, or _Float16 (values
+// rounded to nearest fp16 on store; every operation on them is fp32).
+template <class L, int COLL, int ST, int FM, int ADDR, class LT>
 __global__ void __launch_bounds__(kBlock)
 k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f,
-          const float* __restrict__ lam_in, float* __restrict__ lam_out,
+          const LT* __restrict__ lam_in, LT* __restrict__ lam_out,
           float* __restrict__ dF, double* __restrict__ part) {
   using T = typename Codec<ST>::T;
   constexpr int Q = L::Q;
   constexpr uint32_t TS = sizeof(T);
+  constexpr uint32_t LS = sizeof(LT);
   Lane l;
   const bool live = lane_setup<TS>(a, remap_block((int)blockIdx.x, a), l);
   float dsp = 0.f, dsm = 0.f;
@@ -156,15 +159,17 @@ k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f,
     const char* fb = reinterpret_cast<const char*>(f);
     const char* lib = reinterpret_cast<const char*>(lam_in);
     char* lob = reinterpret_cast<char*>(lam_out);
-    const size_t lplane = N * sizeof(float);
-    const uint32_t q0 = l.n * 4u, qm = l.nm * 4u, qp = l.np * 4u;  // fp32 lane offsets
+    const size_t lplane = N * LS;      // adjoint-state plane
+    const size_t fplane = N * 4u;      // fp32 force / dF plane
+    const uint32_t q0 = l.n * LS, qm = l.nm * LS, qp = l.np * LS;  // adjoint-state lane offsets
+    const uint32_t f0 = l.n * 4u;                                   // fp32 lane offset
     // every load (pulled f, own lambda, force, dF) is issued before the first wait
     float h[Q], lam[Q];
     T t[Q];
     float lid[Q];
     gather_issue<L, ST, PYLBM_AB, ADDR>(a, l, fb, N * TS, t, lid);
 #pragma unroll
-    for (int i = 0; i < Q; ++i) lam[i] = ld_pl<float, ADDR>(lib, lplane, i, 0, q0);
+    for (int i = 0; i < Q; ++i) lam[i] = (float)ld_pl<LT, ADDR>(lib, lplane, i, 0, q0);
     constexpr bool FF = FM == 2;
     float F[3] = {a.fx, a.fy, a.fz};
     const char* ffb = reinterpret_cast<const char*>(a.ff);
@@ -172,9 +177,9 @@ k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f,
     float dF0[3] = {0.f, 0.f, 0.f};
     if (FF) {
 #pragma unroll
-      for (int d = 0; d < L::D; ++d) F[d] = ld_pl<float, ADDR>(ffb, lplane, d, 0, q0);
+      for (int d = 0; d < L::D; ++d) F[d] = ld_pl<float, ADDR>(ffb, fplane, d, 0, f0);
 #pragma unroll
-      for (int d = 0; d < L::D; ++d) dF0[d] = ld_pl<float, ADDR>(dFb, lplane, d, 0, q0);
+      for (int d = 0; d < L::D; ++d) dF0[d] = ld_pl<float, ADDR>(dFb, fplane, d, 0, f0);
     }
     gather_finish<L, ST, PYLBM_AB, ADDR>(l, fb, N * TS, t, lid, h);
     float mu[Q], dFn[3];
@@ -183,25 +188,26 @@ k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f,
     // slot opp(i) on a wall link
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
+      const LT m = (LT)mu[i];
       if (i == 0) {
-        st_pl<2, ADDR>(lob, lplane, 0, 0, q0, mu[0]);
+        st_pl<2, ADDR>(lob, lplane, 0, 0, q0, m);
         continue;
       }
       const int cx = L::c(i, 0);
       const RowLink r = row_link<L>(a, l.y, l.z, i, -1);
       if (r.wall) {
-        st_pl<2, ADDR>(lob, lplane, L::opp(i), 0, q0, mu[i]);
+        st_pl<2, ADDR>(lob, lplane, L::opp(i), 0, q0, m);
         continue;
       }
       const uint32_t off = cx > 0 ? qm : (cx < 0 ? qp : q0);
       if ((cx > 0 && l.wall_lo) || (cx < 0 && l.wall_hi))
-        st_pl<2, ADDR>(lob, lplane, L::opp(i), 0, q0, mu[i]);
+        st_pl<2, ADDR>(lob, lplane, L::opp(i), 0, q0, m);
       else
-        st_pl<2, ADDR>(lob, lplane, i, r.delta * (ptrdiff_t)4, off, mu[i]);
+        st_pl<2, ADDR>(lob, lplane, i, r.delta * (ptrdiff_t)LS, off, m);
     }
     if (FF) {
 #pragma unroll
-      for (int d = 0; d < L::D; ++d) st_pl<0, ADDR>(dFb, lplane, d, 0, q0, dF0[d] + dFn[d]);
+      for (int d = 0; d < L::D; ++d) st_pl<0, ADDR>(dFb, fplane, d, 0, f0, dF0[d] + dFn[d]);
     }
   }
   // per-workgroup partial sums of dL/ds+ and dL/ds- (deterministic: one slot per group)
@@ -228,10 +234,10 @@ k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f,
 
 // Terminal adjoint state of L(u(f)), u = (sum c f - F/2)/rho post-collision:
 //   lam_k = sum_a G_a (c_ka - u_a) / rho,   dF_a = -G_a / (2 rho)   (G = dL/du)
-template <class L, int ST, bool FF>
+template <class L, int ST, bool FF, class LT>
 __global__ void __launch_bounds__(kBlock)
 k_adjoint_seed(const KArgs a, const typename Codec<ST>::T* __restrict__ f,
-               const float* __restrict__ G, float* __restrict__ lam, float* __restrict__ dF) {
+               const float* __restrict__ G, LT* __restrict__ lam, float* __restrict__ dF) {
   using C = Codec<ST>;
   const int n = blockIdx.x * kBlock + threadIdx.x;
   if (n >= a.N) return;
@@ -261,7 +267,7 @@ k_adjoint_seed(const KArgs a, const typename Codec<ST>::T* __restrict__ f,
     float m = 0.f;
 #pragma unroll
     for (int d = 0; d < L::D; ++d) m += Gv[d] * ((float)L::c(k, d) - u[d]);
-    lam[(size_t)k * N + n] = m * inv;
+    lam[(size_t)k * N + n] = (LT)(m * inv);
   }
   if (FF) {
 #pragma unroll
@@ -271,10 +277,10 @@ k_adjoint_seed(const KArgs a, const typename Codec<ST>::T* __restrict__ f,
 
 // Gradient of the initial equilibrium f0 = feq(rho0, u0) (float64 fields) with respect to
 // u0 and rho0, given lam0 = dL/df0.
-template <class L>
+template <class L, class LT>
 __global__ void __launch_bounds__(kBlock)
 k_adjoint_init(int N_, const double* __restrict__ rho0, const double* __restrict__ u0,
-               const float* __restrict__ lam, double* __restrict__ du, double* __restrict__ drho) {
+               const LT* __restrict__ lam, double* __restrict__ du, double* __restrict__ drho) {
   const int n = blockIdx.x * kBlock + threadIdx.x;
   if (n >= N_) return;
   const size_t N = (size_t)N_;
@@ -288,7 +294,7 @@ k_adjoint_init(int N_, const double* __restrict__ rho0, const double* __restrict
   double gu[3] = {0.0, 0.0, 0.0}, gr = 0.0;
 #pragma unroll
   for (int i = 0; i < L::Q; ++i) {
-    const double l = (double)lam[(size_t)i * N + n];
+    const double l = (double)(float)lam[(size_t)i * N + n];
     const double cu = L::c(i, 0) * u[0] + L::c(i, 1) * u[1] + L::c(i, 2) * u[2];
     gr += l * L::w(i) * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * usq);
 #pragma unroll
@@ -302,106 +308,139 @@ k_adjoint_init(int N_, const double* __restrict__ rho0, const double* __restrict
   if (drho) drho[n] = gr;
 }
 
-template <class L, int COLL, int ST>
-int adj_step_t(const KArgs& a, bool field, const void* f, const float* lin, float* lout,
+template <class L, int COLL, int ST, class LT>
+int adj_step_t(const KArgs& a, bool field, const void* f, const void* lin, void* lout,
                float* dF, double* part, hipStream_t s) {
   using T = typename Codec<ST>::T;
   const dim3 grid((unsigned)a.nblk), block(kBlock);
   const bool uniform = a.fx != 0.0f || a.fy != 0.0f || a.fz != 0.0f;
-  // one descriptor per array when the adjoint state (fp32, the largest array) fits 4 GiB
+  const T* fs = (const T*)f;
+  const LT* li = (const LT*)lin;
+  LT* lo = (LT*)lout;
+  // one descriptor per array when a q-plane fp32 array (the largest the kernel can touch)
+  // spans less than 4 GiB
   const bool small = (uint64_t)L::Q * (uint64_t)a.N * 4u < ((uint64_t)1 << 32);
   if (small) {
     if (field)
-      k_adjoint<L, COLL, ST, 2, 1><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF, part);
+      k_adjoint<L, COLL, ST, 2, 1, LT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
     else if (uniform)
-      k_adjoint<L, COLL, ST, 1, 1><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF, part);
+      k_adjoint<L, COLL, ST, 1, 1, LT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
     else
-      k_adjoint<L, COLL, ST, 0, 1><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF, part);
+      k_adjoint<L, COLL, ST, 0, 1, LT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
   } else {
     if (field)
-      k_adjoint<L, COLL, ST, 2, 0><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF,
-                                                              part);
+      k_adjoint<L, COLL, ST, 2, 0, LT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
     else if (uniform)
-      k_adjoint<L, COLL, ST, 1, 0><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF,
-                                                              part);
+      k_adjoint<L, COLL, ST, 1, 0, LT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
     else
-      k_adjoint<L, COLL, ST, 0, 0><<<grid, block, 0, s>>>(a, (const T*)f, lin, lout, dF,
-                                                              part);
+      k_adjoint<L, COLL, ST, 0, 0, LT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
   }
   return (int)hipGetLastError();
 }
 
-template <class L, int COLL>
-int adj_step_s(const KArgs& a, int storage, bool field, const void* f, const float* lin,
-               float* lout, float* dF, double* part, hipStream_t s) {
+template <class L, int COLL, class LT>
+int adj_step_s(const KArgs& a, int storage, bool field, const void* f, const void* lin,
+               void* lout, float* dF, double* part, hipStream_t s) {
   switch (storage) {
-    case PYLBM_F32: return adj_step_t<L, COLL, PYLBM_F32>(a, field, f, lin, lout, dF, part, s);
-    case PYLBM_F16: return adj_step_t<L, COLL, PYLBM_F16>(a, field, f, lin, lout, dF, part, s);
-    case PYLBM_BF16: return adj_step_t<L, COLL, PYLBM_BF16>(a, field, f, lin, lout, dF, part, s);
-    case PYLBM_F32S: return adj_step_t<L, COLL, PYLBM_F32S>(a, field, f, lin, lout, dF, part, s);
+    case PYLBM_F32:
+      return adj_step_t<L, COLL, PYLBM_F32, LT>(a, field, f, lin, lout, dF, part, s);
+    case PYLBM_F16:
+      return adj_step_t<L, COLL, PYLBM_F16, LT>(a, field, f, lin, lout, dF, part, s);
+    case PYLBM_BF16:
+      return adj_step_t<L, COLL, PYLBM_BF16, LT>(a, field, f, lin, lout, dF, part, s);
+    case PYLBM_F32S:
+      return adj_step_t<L, COLL, PYLBM_F32S, LT>(a, field, f, lin, lout, dF, part, s);
   }
   return PYLBM_EINVAL;
 }
 
-template <class L>
+template <class L, class LT>
 int adj_step_l(const KArgs& a, int coll, int storage, bool field, const void* f,
-               const float* lin, float* lout, float* dF, double* part, hipStream_t s) {
+               const void* lin, void* lout, float* dF, double* part, hipStream_t s) {
   if (coll == PYLBM_BGK)
-    return adj_step_s<L, PYLBM_BGK>(a, storage, field, f, lin, lout, dF, part, s);
+    return adj_step_s<L, PYLBM_BGK, LT>(a, storage, field, f, lin, lout, dF, part, s);
   if (coll == PYLBM_TRT)
-    return adj_step_s<L, PYLBM_TRT>(a, storage, field, f, lin, lout, dF, part, s);
+    return adj_step_s<L, PYLBM_TRT, LT>(a, storage, field, f, lin, lout, dF, part, s);
   return PYLBM_ENOTSUP;
 }
 
-template <class L, int ST>
-int seed_t(const KArgs& a, bool field, const void* f, const float* G, float* lam, float* dF,
+template <class L, int ST, class LT>
+int seed_t(const KArgs& a, bool field, const void* f, const float* G, void* lam, float* dF,
            hipStream_t s) {
   using T = typename Codec<ST>::T;
   const dim3 grid((unsigned)((a.N + kBlock - 1) / kBlock)), block(kBlock);
   if (field)
-    k_adjoint_seed<L, ST, true><<<grid, block, 0, s>>>(a, (const T*)f, G, lam, dF);
+    k_adjoint_seed<L, ST, true, LT><<<grid, block, 0, s>>>(a, (const T*)f, G, (LT*)lam, dF);
   else
-    k_adjoint_seed<L, ST, false><<<grid, block, 0, s>>>(a, (const T*)f, G, lam, dF);
+    k_adjoint_seed<L, ST, false, LT><<<grid, block, 0, s>>>(a, (const T*)f, G, (LT*)lam, dF);
   return (int)hipGetLastError();
 }
 
-template <class L>
-int seed_l(const KArgs& a, int storage, bool field, const void* f, const float* G, float* lam,
+template <class L, class LT>
+int seed_l(const KArgs& a, int storage, bool field, const void* f, const float* G, void* lam,
            float* dF, hipStream_t s) {
   switch (storage) {
-    case PYLBM_F32: return seed_t<L, PYLBM_F32>(a, field, f, G, lam, dF, s);
-    case PYLBM_F16: return seed_t<L, PYLBM_F16>(a, field, f, G, lam, dF, s);
-    case PYLBM_BF16: return seed_t<L, PYLBM_BF16>(a, field, f, G, lam, dF, s);
-    case PYLBM_F32S: return seed_t<L, PYLBM_F32S>(a, field, f, G, lam, dF, s);
+    case PYLBM_F32: return seed_t<L, PYLBM_F32, LT>(a, field, f, G, lam, dF, s);
+    case PYLBM_F16: return seed_t<L, PYLBM_F16, LT>(a, field, f, G, lam, dF, s);
+    case PYLBM_BF16: return seed_t<L, PYLBM_BF16, LT>(a, field, f, G, lam, dF, s);
+    case PYLBM_F32S: return seed_t<L, PYLBM_F32S, LT>(a, field, f, G, lam, dF, s);
   }
   return PYLBM_EINVAL;
+}
+
+template <class L>
+int adj_step_lt(const KArgs& a, int coll, int storage, int lam_storage, bool field,
+                const void* f, const void* lin, void* lout, float* dF, double* part,
+                hipStream_t s) {
+  if (lam_storage == PYLBM_F16)
+    return adj_step_l<L, _Float16>(a, coll, storage, field, f, lin, lout, dF, part, s);
+  return adj_step_l<L, float>(a, coll, storage, field, f, lin, lout, dF, part, s);
+}
+
+template <class L>
+int seed_lt(const KArgs& a, int storage, int lam_storage, bool field, const void* f,
+            const float* G, void* lam, float* dF, hipStream_t s) {
+  if (lam_storage == PYLBM_F16) return seed_l<L, _Float16>(a, storage, field, f, G, lam, dF, s);
+  return seed_l<L, float>(a, storage, field, f, G, lam, dF, s);
+}
+
+template <class L>
+void init_lt(int lam_storage, dim3 grid, int N, const double* rho0, const double* u0,
+             const void* lam, double* du, double* drho, hipStream_t s) {
+  if (lam_storage == PYLBM_F16)
+    k_adjoint_init<L, _Float16><<<grid, dim3(kBlock), 0, s>>>(N, rho0, u0,
+                                                              (const _Float16*)lam, du, drho);
+  else
+    k_adjoint_init<L, float><<<grid, dim3(kBlock), 0, s>>>(N, rho0, u0, (const float*)lam, du,
+                                                           drho);
 }
 
 }  // namespace
 
 int adjoint_partials(const KArgs& a) { return 2 * a.nblk; }
 
-int launch_adjoint_step(int lattice, const KArgs& a, int coll, int storage, bool field,
-                        const void* f, const float* lin, float* lout, float* dF, double* part,
+int launch_adjoint_step(int lattice, const KArgs& a, int coll, int storage, int lam_storage,
+                        bool field, const void* f, const void* lin, void* lout, float* dF,
+                        double* part, hipStream_t s) {
+  if (lattice == PYLBM_D2Q9)
+    return adj_step_lt<D2Q9>(a, coll, storage, lam_storage, field, f, lin, lout, dF, part, s);
+  return adj_step_lt<D3Q19>(a, coll, storage, lam_storage, field, f, lin, lout, dF, part, s);
+}
+
+int launch_adjoint_seed(int lattice, const KArgs& a, int storage, int lam_storage, bool field,
+                        const void* f, const float* G, void* lam, float* dF, hipStream_t s) {
+  if (lattice == PYLBM_D2Q9) return seed_lt<D2Q9>(a, storage, lam_storage, field, f, G, lam, dF, s);
+  return seed_lt<D3Q19>(a, storage, lam_storage, field, f, G, lam, dF, s);
+}
+
+int launch_adjoint_init(int lattice, int lam_storage, int N, const double* rho0,
+                        const double* u0, const void* lam, double* du, double* drho,
                         hipStream_t s) {
+  const dim3 grid((unsigned)((N + kBlock - 1) / kBlock));
   if (lattice == PYLBM_D2Q9)
-    return adj_step_l<D2Q9>(a, coll, storage, field, f, lin, lout, dF, part, s);
-  return adj_step_l<D3Q19>(a, coll, storage, field, f, lin, lout, dF, part, s);
-}
-
-int launch_adjoint_seed(int lattice, const KArgs& a, int storage, bool field, const void* f,
-                        const float* G, float* lam, float* dF, hipStream_t s) {
-  if (lattice == PYLBM_D2Q9) return seed_l<D2Q9>(a, storage, field, f, G, lam, dF, s);
-  return seed_l<D3Q19>(a, storage, field, f, G, lam, dF, s);
-}
-
-int launch_adjoint_init(int lattice, int N, const double* rho0, const double* u0,
-                        const float* lam, double* du, double* drho, hipStream_t s) {
-  const dim3 grid((unsigned)((N + kBlock - 1) / kBlock)), block(kBlock);
-  if (lattice == PYLBM_D2Q9)
-    k_adjoint_init<D2Q9><<<grid, block, 0, s>>>(N, rho0, u0, lam, du, drho);
+    init_lt<D2Q9>(lam_storage, grid, N, rho0, u0, lam, du, drho, s);
   else
-    k_adjoint_init<D3Q19><<<grid, block, 0, s>>>(N, rho0, u0, lam, du, drho);
+    init_lt<D3Q19>(lam_storage, grid, N, rho0, u0, lam, du, drho, s);
   return (int)hipGetLastError();
 }
 

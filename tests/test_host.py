@@ -203,7 +203,7 @@ def _params(**over) -> _lib.Params:
     (dict(nz=2, n_nodes=256), "nz == 1"),
     (dict(nx=0, n_nodes=0), "sizes"),
     (dict(aa=2), "aa must be"),
-    (dict(reserved=1), "reserved"),
+    (dict(lam_storage=2), "lam_storage"),
     (dict(outlet=2), "outlet must be"),
     (dict(aa=1, links=0x3000), "two-buffer"),
     (dict(lattice=_lib.D3Q19, nz=2, n_nodes=256, outlet=1), "D2Q9 only"),

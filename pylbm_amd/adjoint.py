@@ -80,6 +80,9 @@ class RolloutConfig:
     storage: str = "f32s"
     #: element type of the adjoint states between steps: "f32" or "f16" (fp32 arithmetic)
     lam_storage: str = "f32"
+    #: recompute the next checkpoint segment on a second stream during each segment's
+    #: adjoint (False: one stream, one after the other; the gradients are the same)
+    overlap: bool = True
 
     def __post_init__(self) -> None:
         p = self.problem
@@ -212,22 +215,51 @@ class _RolloutFn(torch.autograd.Function):
                   _s())
         cur = 0  # index of the lam buffer holding the current adjoint state
         which = ctypes.c_int(0)
-        seg_buf: list[torch.Tensor] = []
+        # Segment k is recomputed from checkpoint k into buffer set k % 2 on a second stream
+        # (default priority) while the main stream runs the adjoint of segment k + 1: the two
+        # are independent, and on small grids neither fills the GPU alone.  Every adjoint
+        # step (and so every dF / partial-sum accumulation) stays on the main stream in
+        # reverse order, so the gradients do not depend on the overlap.
+        main = torch.cuda.current_stream(eng.dev)
+        side = torch.cuda.Stream(device=eng.dev) if cfg.overlap else main
+        side.wait_stream(main)  # the checkpoints and the seed
+        seg_sets: list[list[torch.Tensor]] = [[], []]
+        ready: dict[int, torch.cuda.Event] = {}
+        done: dict[int, torch.cuda.Event] = {}
+
+        def recompute(k: int) -> None:
+            n = min(C, T - k * C)
+            buf = seg_sets[k % 2]
+            while len(buf) < n - 1:
+                buf.append(eng.state())
+            if k + 2 in done:  # the adjoint of segment k + 2 read this buffer set
+                side.wait_event(done[k + 2])
+            with torch.cuda.stream(side):
+                _lib.call("pylbm_hip_run_store", eng.ref, _p(ctx.ckpts[k]),
+                          _ptr_array(buf[:n - 1]), n - 1, int(side.cuda_stream))
+            ev = torch.cuda.Event()
+            ev.record(side)
+            ready[k] = ev
+
         for k in range(K - 1, -1, -1):
             n = min(C, T - k * C)
             if k == K - 1:
                 states = ctx.last
-            else:  # recompute f^{kC+1} .. f^{kC+n-1} from the checkpoint
-                while len(seg_buf) < n - 1:
-                    seg_buf.append(eng.state())
-                _lib.call("pylbm_hip_run_store", eng.ref, _p(ctx.ckpts[k]),
-                          _ptr_array(seg_buf[:n - 1]), n - 1, _s())
-                states = [ctx.ckpts[k]] + seg_buf[:n - 1]
+            else:
+                main.wait_event(ready.pop(k))
+                states = [ctx.ckpts[k]] + seg_sets[k % 2][:n - 1]
+            if k >= 1:  # enqueued before this segment's adjoint, so the two overlap
+                recompute(k - 1)
             _lib.call("pylbm_hip_adjoint_run", eng.ref, _ptr_array(states), n, _p(lam[cur]),
-                      _p(lam[1 - cur]), _p(dF), _p(part), ctypes.byref(which), _s())
+                      _p(lam[1 - cur]), _p(dF), _p(part), ctypes.byref(which),
+                      int(main.cuda_stream))
             cur = cur if which.value == 0 else 1 - cur
+            ev = torch.cuda.Event()
+            ev.record(main)
+            done[k] = ev
             if k == K - 1:
                 ctx.last = None
+        main.wait_stream(side)  # the segment buffers are freed after the side stream's work
         lam0 = lam[cur]
         need_F, need_tau, need_u, need_rho = ctx.needs
         du0 = drho0 = None

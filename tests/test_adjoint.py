@@ -258,6 +258,29 @@ def test_channel_gradients_vs_float64_autograd_and_fd():
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("lam_storage", ["f32", "f16"])
+def test_overlapped_recompute_gives_identical_gradients(lam_storage):
+    """The backward pass recomputes segment k - 1 on a second stream during the adjoint of
+    segment k; the adjoint steps stay in order on one stream, so the gradients equal the
+    one-stream backward pass bit for bit (T = 250, C = 40: 7 segments, a short last one)."""
+    from pylbm_amd.adjoint import RolloutConfig, rollout, velocity_loss
+
+    nx, ny, T = 96, 40, 250
+    ff, ut = _channel_setup(nx, ny, seed=7)
+    kw = dict(shape=(nx, ny), tau=0.75, periodic=(True, False), collision="trt")
+    p = Problem(**kw, force_field=ff)
+    grads = []
+    for overlap in (True, False, True):
+        cfg = RolloutConfig(p, T, checkpoint=40, lam_storage=lam_storage, overlap=overlap)
+        F = _dev(ff).float().requires_grad_(True)
+        tau = torch.tensor(0.75, dtype=torch.float64, device="cuda", requires_grad=True)
+        velocity_loss(rollout(cfg, F, tau), _dev(ut)).backward()
+        grads.append((F.grad.cpu().numpy(), float(tau.grad)))
+    for g in grads[1:]:
+        assert np.array_equal(g[0], grads[0][0]) and g[1] == grads[0][1]
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("storage,lam_storage,bound", [
     ("f32s", "f32", 1e-4), ("f32", "f32", 1e-3), ("f16", "f32", 5e-2),
     # fp16 adjoint states: the float64 adjoint with its states rounded to fp16 after every

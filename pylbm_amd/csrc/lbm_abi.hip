@@ -140,6 +140,8 @@ int make_kargs(const pylbm_params* p, KArgs& a, const char* what) {
   const bool cached = 2 * q * (int64_t)p->n_nodes * item <= ((int64_t)256 << 20);
   set_xcd_order(a, cached ? kOrderSlabs
                           : (p->lattice == PYLBM_D3Q19 ? kOrderChunk16 : kOrderRun));
+  a.reverse = 0;
+  a.resident = cached ? 1 : 0;
   a.div_nseg = make_fastdiv((uint32_t)a.nseg);
   a.div_ny = make_fastdiv((uint32_t)p->ny);
   a.ulid = p->ulid;
@@ -218,13 +220,16 @@ int pylbm_hip_run(const pylbm_params* p, void* a_buf, void* b_buf, int64_t nstep
   if (!a_buf) return fail(PYLBM_EINVAL, "pylbm_hip_run: NULL buffer");
   KArgs a;
   if (int e = make_kargs(p, a, "pylbm_hip_run")) return e;
+  KArgs ar = a;
+  ar.reverse = 1;
   hipStream_t s = (hipStream_t)stream;
   if (p->aa) {
     if (nsteps % 2) return fail(PYLBM_EINVAL, "pylbm_hip_run: AA advances in pairs of steps");
+    // AA: the sweep direction alternates every launch (remap_block)
     for (int64_t k = 0; k < nsteps; k += 2) {
       if (int e = step_impl(p, a, a_buf, a_buf, PYLBM_AA_ODD, s))
         return hip_fail(e, "pylbm_hip_run (aa-odd)");
-      if (int e = step_impl(p, a, a_buf, a_buf, PYLBM_AA_EVEN, s))
+      if (int e = step_impl(p, ar, a_buf, a_buf, PYLBM_AA_EVEN, s))
         return hip_fail(e, "pylbm_hip_run (aa-even)");
     }
     return 0;
@@ -233,8 +238,11 @@ int pylbm_hip_run(const pylbm_params* p, void* a_buf, void* b_buf, int64_t nstep
     return fail(PYLBM_EINVAL, "pylbm_hip_run: two-buffer stepping needs a distinct b");
   void* src = a_buf;
   void* dst = b_buf;
+  // two buffers: the direction alternates for resident states only (a large state gains
+  // nothing, its stores stream past the Infinity Cache, and measured 1-2% slower)
   for (int64_t k = 0; k < nsteps; ++k) {
-    if (int e = step_impl(p, a, src, dst, PYLBM_AB, s)) return hip_fail(e, "pylbm_hip_run");
+    if (int e = step_impl(p, ((k & 1) && a.resident) ? ar : a, src, dst, PYLBM_AB, s))
+      return hip_fail(e, "pylbm_hip_run");
     void* t = src;
     src = dst;
     dst = t;
@@ -250,12 +258,15 @@ int pylbm_hip_run_store(const pylbm_params* p, const void* src, void* const* dst
   if (nsteps > 0 && (!src || !dst)) return fail(PYLBM_EINVAL, "pylbm_hip_run_store: NULL buffer");
   KArgs a;
   if (int e = make_kargs(p, a, "pylbm_hip_run_store")) return e;
+  KArgs ar = a;
+  ar.reverse = 1;
   const void* cur = src;
   for (int64_t k = 0; k < nsteps; ++k) {
     if (!dst[k] || dst[k] == cur)
       return fail(PYLBM_EINVAL, "pylbm_hip_run_store: dst[%lld] is NULL or equals its source",
                   (long long)k);
-    if (int e = step_impl(p, a, cur, dst[k], PYLBM_AB, (hipStream_t)stream))
+    if (int e = step_impl(p, ((k & 1) && a.resident) ? ar : a, cur, dst[k], PYLBM_AB,
+                          (hipStream_t)stream))
       return hip_fail(e, "pylbm_hip_run_store");
     cur = dst[k];
   }
@@ -399,12 +410,17 @@ int pylbm_hip_adjoint_run(const pylbm_params* p, const void* const* states, int6
     return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_run: a force field needs dF");
   KArgs a;
   if (int e = make_kargs(p, a, "pylbm_hip_adjoint_run")) return e;
+  KArgs ar = a;
+  ar.reverse = 1;
   void* lin = lam_a;
   void* lout = lam_b;
   for (int64_t k = nsteps - 1; k >= 0; --k) {
     if (!states[k]) return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_run: states[%lld] is NULL",
                                 (long long)k);
-    if (int e = launch_adjoint_step(p->lattice, a, p->collision, p->storage, p->lam_storage,
+    // the sweep direction alternates every step for resident states (remap_block)
+    const bool rev = ((nsteps - 1 - k) & 1) && a.resident;
+    if (int e = launch_adjoint_step(p->lattice, rev ? ar : a, p->collision,
+                                    p->storage, p->lam_storage,
                                     p->force_field != nullptr, states[k], lin, lout, dF,
                                     dtau_part, (hipStream_t)stream))
       return hip_fail(e, "pylbm_hip_adjoint_run");

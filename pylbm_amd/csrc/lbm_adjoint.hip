@@ -142,7 +142,7 @@ __device__ __forceinline__ float wave_sum_f(float v) {
 
 // LT is the element type of the adjoint states lam_in / lam_out: float, or _Float16 (values
 // rounded to nearest fp16 on store; every operation on them is fp32).
-template <class L, int COLL, int ST, int FM, int ADDR, class LT>
+template <class L, int COLL, int ST, int FM, int ADDR, class LT, int SP>
 __global__ void __launch_bounds__(kBlock)
 k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f,
           const LT* __restrict__ lam_in, LT* __restrict__ lam_out,
@@ -190,20 +190,20 @@ k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f,
     for (int i = 0; i < Q; ++i) {
       const LT m = (LT)mu[i];
       if (i == 0) {
-        st_pl<2, ADDR>(lob, lplane, 0, 0, q0, m);
+        st_pl<SP, ADDR>(lob, lplane, 0, 0, q0, m);
         continue;
       }
       const int cx = L::c(i, 0);
       const RowLink r = row_link<L>(a, l.y, l.z, i, -1);
       if (r.wall) {
-        st_pl<2, ADDR>(lob, lplane, L::opp(i), 0, q0, m);
+        st_pl<SP, ADDR>(lob, lplane, L::opp(i), 0, q0, m);
         continue;
       }
       const uint32_t off = cx > 0 ? qm : (cx < 0 ? qp : q0);
       if ((cx > 0 && l.wall_lo) || (cx < 0 && l.wall_hi))
-        st_pl<2, ADDR>(lob, lplane, L::opp(i), 0, q0, m);
+        st_pl<SP, ADDR>(lob, lplane, L::opp(i), 0, q0, m);
       else
-        st_pl<2, ADDR>(lob, lplane, i, r.delta * (ptrdiff_t)LS, off, m);
+        st_pl<SP, ADDR>(lob, lplane, i, r.delta * (ptrdiff_t)LS, off, m);
     }
     if (FF) {
 #pragma unroll
@@ -320,20 +320,29 @@ int adj_step_t(const KArgs& a, bool field, const void* f, const void* lin, void*
   // one descriptor per array when a q-plane fp32 array (the largest the kernel can touch)
   // spans less than 4 GiB
   const bool small = (uint64_t)L::Q * (uint64_t)a.N * 4u < ((uint64_t)1 << 32);
-  if (small) {
+  // SP: temporal adjoint-state stores when the forward state is resident in the Infinity
+  // Cache (the next adjoint step re-reads them), non-temporal otherwise
+  if (small && a.resident) {
     if (field)
-      k_adjoint<L, COLL, ST, 2, 1, LT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
+      k_adjoint<L, COLL, ST, 2, 1, LT, 0><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
     else if (uniform)
-      k_adjoint<L, COLL, ST, 1, 1, LT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
+      k_adjoint<L, COLL, ST, 1, 1, LT, 0><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
     else
-      k_adjoint<L, COLL, ST, 0, 1, LT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
+      k_adjoint<L, COLL, ST, 0, 1, LT, 0><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
+  } else if (small) {
+    if (field)
+      k_adjoint<L, COLL, ST, 2, 1, LT, 2><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
+    else if (uniform)
+      k_adjoint<L, COLL, ST, 1, 1, LT, 2><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
+    else
+      k_adjoint<L, COLL, ST, 0, 1, LT, 2><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
   } else {
     if (field)
-      k_adjoint<L, COLL, ST, 2, 0, LT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
+      k_adjoint<L, COLL, ST, 2, 0, LT, 2><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
     else if (uniform)
-      k_adjoint<L, COLL, ST, 1, 0, LT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
+      k_adjoint<L, COLL, ST, 1, 0, LT, 2><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
     else
-      k_adjoint<L, COLL, ST, 0, 0, LT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
+      k_adjoint<L, COLL, ST, 0, 0, LT, 2><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
   }
   return (int)hipGetLastError();
 }

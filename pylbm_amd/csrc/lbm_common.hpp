@@ -133,6 +133,9 @@ struct KArgs {
   int per;     // bit a: axis a periodic
   int order;   // workgroup order over the XCDs (host policy, xcd_chunk)
   int chunk;   // consecutive workgroups an XCD takes per turn (0: plain round-robin)
+  int reverse; // 1: work items in reverse order (the sweep direction of this launch)
+  int resident; // 1: the state fits in the 256 MiB Infinity Cache (slab order, temporal
+                // stores, alternating sweeps)
   FastDiv div_nseg, div_ny, div_chunk;
   float ulid;
   float fx, fy, fz;
@@ -149,7 +152,18 @@ struct KArgs {
 // item ((i / C) * 8 + x) * C + i % C, with x = b % 8 and i = b / 8.  C = nblk / 8 gives
 // every XCD its own slab (neighbouring segments and rows share one L2); small C keeps the
 // eight XCDs sweeping one window of rows together; C = 0 leaves the dispatch order.
+//
+// reverse = 1 runs each XCD's sequence of work items backwards (when nblk is a multiple of
+// 8, as it always is with chunk > 0, every XCD keeps its own items, so a slab stays with
+// the L2 that holds it; otherwise the whole order is reversed).  Multi-step loops
+// alternate the direction where it pays (the AA pattern, states resident in the 256 MiB
+// Infinity Cache): a launch then starts on the rows the previous one wrote last, which
+// the caches still hold.
 __device__ __forceinline__ int remap_block(int b, const KArgs& a) {
+  if (a.reverse) {
+    if ((a.nblk & 7) == 0) b = ((((a.nblk >> 3) - 1) - (b >> 3)) << 3) | (b & 7);
+    else b = a.nblk - 1 - b;
+  }
   if (a.chunk == 0) return b;
   const int x = b & 7, i = b >> 3;
   const int turn = (int)a.div_chunk.div((uint32_t)i);

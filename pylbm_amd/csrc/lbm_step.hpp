@@ -21,7 +21,9 @@ enum ForceMode { kForceNone = 0, kForceUniform = 1, kForceField = 2 };
 // Store the post-collision deviations d of one node for pattern PAT
 // (pylbm/codegen.py:10-16): AB -> B[i, x]; AA even -> A[opp(i), x]; AA odd -> A[i, x + c_i],
 // or the own slot opp(i) on a wall link (minus the lid term through the lid).
-template <class L, int ST, int PAT, int ADDR>
+// SP: cache policy of the two-buffer and AA-even stores (2 non-temporal for states larger
+// than the Infinity Cache, 0 for resident states, whose next launch re-reads them).
+template <class L, int ST, int PAT, int ADDR, int SP>
 __device__ __forceinline__ void store_node(const KArgs& a, const Lane& l, char* Bb,
                                            size_t plane, const float (&d)[L::Q]) {
   using C = Codec<ST>;
@@ -32,11 +34,11 @@ __device__ __forceinline__ void store_node(const KArgs& a, const Lane& l, char* 
   for (int i = 0; i < Q; ++i) {
     const float v = C::shifted ? d[i] : d[i] + (float)L::w(i);
     if (PAT == PYLBM_AB) {
-      st_pl<2, ADDR>(Bb, plane, i, 0, l.o0, C::st(v));
+      st_pl<SP, ADDR>(Bb, plane, i, 0, l.o0, C::st(v));
       continue;
     }
     if (PAT == PYLBM_AA_EVEN) {
-      st_pl<2, ADDR>(Bb, plane, L::opp(i), 0, l.o0, C::st(v));
+      st_pl<SP, ADDR>(Bb, plane, L::opp(i), 0, l.o0, C::st(v));
       continue;
     }
     if (i == 0) {
@@ -64,7 +66,7 @@ __device__ __forceinline__ void store_node(const KArgs& a, const Lane& l, char* 
 // k * kBlock + t for k < NPL, so every wave access stays one contiguous, aligned run of 64
 // elements; the loads of all NPL nodes are issued before the first collision, which
 // doubles the bytes a wave keeps in flight when NPL = 2.
-template <class L, int COLL, int ST, int PAT, int FM, int ADDR, int NPL>
+template <class L, int COLL, int ST, int PAT, int FM, int ADDR, int NPL, int SP>
 __global__ void __launch_bounds__(kBlock)
 k_step(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T* B) {
   using C = Codec<ST>;
@@ -116,7 +118,7 @@ k_step(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T* B) 
     float d[Q];
     collide_node<L, COLL, FM != kForceNone>(h, d, F[k], a.r0, a.r1, a.r2, a.r3);
     if (k > 0 && !act[k]) break;
-    store_node<L, ST, PAT, ADDR>(a, l[k], Bb, plane, d);
+    store_node<L, ST, PAT, ADDR, SP>(a, l[k], Bb, plane, d);
   }
 }
 
@@ -149,34 +151,33 @@ inline KArgs with_npl(const KArgs& a, int npl) {
   return b;
 }
 
-template <class L, int COLL, int ST, int PAT, int ADDR, int NPL>
+template <class L, int COLL, int ST, int PAT, int ADDR, int NPL, int SP>
 void launch_step_n(const KArgs& a0, int fmode, const void* src, void* dst, hipStream_t s) {
   using T = typename Codec<ST>::T;
   const KArgs a = with_npl(a0, NPL);
   const dim3 grid((unsigned)a.nblk), block(kBlock);
+  const T* in = (const T*)src;
+  T* out = (T*)dst;
   if (fmode == kForceField)
-    k_step<L, COLL, ST, PAT, kForceField, ADDR, NPL><<<grid, block, 0, s>>>(a, (const T*)src,
-                                                                             (T*)dst);
+    k_step<L, COLL, ST, PAT, kForceField, ADDR, NPL, SP><<<grid, block, 0, s>>>(a, in, out);
   else if (fmode == kForceUniform)
-    k_step<L, COLL, ST, PAT, kForceUniform, ADDR, NPL><<<grid, block, 0, s>>>(
-        a, (const T*)src, (T*)dst);
+    k_step<L, COLL, ST, PAT, kForceUniform, ADDR, NPL, SP><<<grid, block, 0, s>>>(a, in, out);
   else
-    k_step<L, COLL, ST, PAT, kForceNone, ADDR, NPL><<<grid, block, 0, s>>>(a, (const T*)src,
-                                                                            (T*)dst);
+    k_step<L, COLL, ST, PAT, kForceNone, ADDR, NPL, SP><<<grid, block, 0, s>>>(a, in, out);
 }
 
 // Several nodes per lane only pay on rows at least one full segment long; shorter rows
 // would leave the extra node slots idle and keep their register cost.
-template <class L, int COLL, int ST, int PAT, int ADDR>
+template <class L, int COLL, int ST, int PAT, int ADDR, int SP>
 void launch_step_f(const KArgs& a, int fmode, const void* src, void* dst, hipStream_t s) {
   constexpr int NPL = nodes_per_lane<L, ST, PAT>();
   if constexpr (NPL > 1) {
     if (a.nx >= kBlock * NPL) {
-      launch_step_n<L, COLL, ST, PAT, ADDR, NPL>(a, fmode, src, dst, s);
+      launch_step_n<L, COLL, ST, PAT, ADDR, NPL, SP>(a, fmode, src, dst, s);
       return;
     }
   }
-  launch_step_n<L, COLL, ST, PAT, ADDR, 1>(a, fmode, src, dst, s);
+  launch_step_n<L, COLL, ST, PAT, ADDR, 1, SP>(a, fmode, src, dst, s);
 }
 
 // Addressing mode of a state array of q planes (plane_ref in lbm_access.hpp): one
@@ -199,11 +200,16 @@ int launch_step_t(const KArgs& a, int fmode, const void* src, void* dst, hipStre
   const uint64_t plane = (uint64_t)a.N * sizeof(typename Codec<ST>::T);
   const uint64_t fbytes = (uint64_t)L::D * (uint64_t)a.N * sizeof(float);
   const int mode = fbytes < ((uint64_t)1 << 32) ? addr_mode(L::Q, plane) : 0;
+  // a state resident in the Infinity Cache (always under 4 GiB) is stored with temporal
+  // stores, so that the next launch finds it there; larger states stream past it
   switch (mode) {
-    case 1: launch_step_f<L, COLL, ST, PAT, 1>(a, fmode, src, dst, s); break;
-    case 8: launch_step_f<L, COLL, ST, PAT, 8>(a, fmode, src, dst, s); break;
-    case 4: launch_step_f<L, COLL, ST, PAT, 4>(a, fmode, src, dst, s); break;
-    default: launch_step_f<L, COLL, ST, PAT, 0>(a, fmode, src, dst, s); break;
+    case 1:
+      if (a.resident) launch_step_f<L, COLL, ST, PAT, 1, 0>(a, fmode, src, dst, s);
+      else launch_step_f<L, COLL, ST, PAT, 1, 2>(a, fmode, src, dst, s);
+      break;
+    case 8: launch_step_f<L, COLL, ST, PAT, 8, 2>(a, fmode, src, dst, s); break;
+    case 4: launch_step_f<L, COLL, ST, PAT, 4, 2>(a, fmode, src, dst, s); break;
+    default: launch_step_f<L, COLL, ST, PAT, 0, 2>(a, fmode, src, dst, s); break;
   }
   return (int)hipGetLastError();
 }

@@ -185,25 +185,39 @@ k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f,
     float mu[Q], dFn[3];
     adjoint_node<L, COLL, FM != 0>(h, F, lam, a.r0, a.r1, mu, dFn, dsp, dsm);
     // push mu_i back along the link it was pulled over: to (i, x - c_i), or to the own
-    // slot opp(i) on a wall link
+    // slot opp(i) on a wall link.  Waves with no lane on an x wall (every wave of a box
+    // periodic in x) take a loop without per-lane branches; the per-lane wall choice costs
+    // each direction a divergent branch and its exec-mask bookkeeping on the scalar unit.
+    if (__ballot(l.wall_lo || l.wall_hi) == 0) {
+      // a y/z wall (uniform) selects the own slot opp(i) by value, without a branch
 #pragma unroll
-    for (int i = 0; i < Q; ++i) {
-      const LT m = (LT)mu[i];
-      if (i == 0) {
-        st_pl<SP, ADDR>(lob, lplane, 0, 0, q0, m);
-        continue;
+      for (int i = 0; i < Q; ++i) {
+        const int cx = L::c(i, 0);
+        const RowLink r = row_link<L>(a, l.y, l.z, i, -1);
+        const bool own = i == 0 || r.wall;
+        st_pl<SP, ADDR>(lob, lplane, own ? L::opp(i) : i, own ? 0 : r.delta * (ptrdiff_t)LS,
+                        own ? q0 : (cx > 0 ? qm : (cx < 0 ? qp : q0)), (LT)mu[i]);
       }
-      const int cx = L::c(i, 0);
-      const RowLink r = row_link<L>(a, l.y, l.z, i, -1);
-      if (r.wall) {
-        st_pl<SP, ADDR>(lob, lplane, L::opp(i), 0, q0, m);
-        continue;
+    } else {
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        const LT m = (LT)mu[i];
+        if (i == 0) {
+          st_pl<SP, ADDR>(lob, lplane, 0, 0, q0, m);
+          continue;
+        }
+        const int cx = L::c(i, 0);
+        const RowLink r = row_link<L>(a, l.y, l.z, i, -1);
+        if (r.wall) {
+          st_pl<SP, ADDR>(lob, lplane, L::opp(i), 0, q0, m);
+          continue;
+        }
+        const uint32_t off = cx > 0 ? qm : (cx < 0 ? qp : q0);
+        if ((cx > 0 && l.wall_lo) || (cx < 0 && l.wall_hi))
+          st_pl<SP, ADDR>(lob, lplane, L::opp(i), 0, q0, m);
+        else
+          st_pl<SP, ADDR>(lob, lplane, i, r.delta * (ptrdiff_t)LS, off, m);
       }
-      const uint32_t off = cx > 0 ? qm : (cx < 0 ? qp : q0);
-      if ((cx > 0 && l.wall_lo) || (cx < 0 && l.wall_hi))
-        st_pl<SP, ADDR>(lob, lplane, L::opp(i), 0, q0, m);
-      else
-        st_pl<SP, ADDR>(lob, lplane, i, r.delta * (ptrdiff_t)LS, off, m);
     }
     if (FF) {
 #pragma unroll

@@ -30,6 +30,23 @@ __device__ __forceinline__ void store_node(const KArgs& a, const Lane& l, char* 
   using T = typename C::T;
   constexpr int Q = L::Q;
   constexpr uint32_t TS = sizeof(T);
+  // AA odd, waves with no lane on an x wall (every wave of a box periodic in x): each push
+  // goes to x + c_i or, through a y/z wall (uniform, selected by value), to the own slot,
+  // without the per-lane branches (each a divergent branch with exec-mask bookkeeping)
+  if (PAT == PYLBM_AA_ODD && __ballot(l.wall_lo || l.wall_hi) == 0) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const float v = C::shifted ? d[i] : d[i] + (float)L::w(i);
+      const int cx = L::c(i, 0);
+      const RowLink r = row_link<L>(a, l.y, l.z, i, +1);
+      const bool own = i == 0 || r.wall;
+      const float lid = (cx != 0 && r.wall && r.top) ? (float)(-6.0 * L::w(i) * cx) * a.ulid
+                                                    : 0.0f;
+      st_pl<0, ADDR>(Bb, plane, own ? L::opp(i) : i, own ? 0 : r.delta * (ptrdiff_t)TS,
+                     own ? l.o0 : (cx > 0 ? l.op : (cx < 0 ? l.om : l.o0)), C::st(v + lid));
+    }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     const float v = C::shifted ? d[i] : d[i] + (float)L::w(i);

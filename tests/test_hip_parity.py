@@ -173,15 +173,19 @@ def test_per_cell_force_field_matches_oracle():
     ff[0] = 1e-5 + 1e-6 * rng.standard_normal((ny, nx))
     ff[1] = 1e-6 * rng.standard_normal((ny, nx))
     ff = ff.astype(np.float32).astype(np.float64)  # the kernels read the field in fp32
-    for coll in ("bgk", "trt"):
-        kw = dict(shape=(nx, ny), tau=0.8, periodic=(True, False), collision=coll,
-                  force_field=ff)
+    # D3Q19 TRT: walls in y and z with a lid, a 3-component field
+    f3 = (1e-5 * rng.standard_normal((3, 6, 8, 12))).astype(np.float32).astype(np.float64)
+    cases = [dict(shape=(nx, ny), tau=0.8, periodic=(True, False), collision=coll,
+                  force_field=ff) for coll in ("bgk", "trt", "mrt")]
+    cases.append(dict(shape=(12, 8, 6), tau=0.7, periodic=(True, False, False), u_lid=0.03,
+                      collision="trt", force_field=f3))
+    for kw in cases:
         ref = R.run(R.Case(**kw), 100)
         for variant in ("fused", "fused-aa"):
             s = _make(variant, Problem(**kw))
             s.step(100)
             err = np.abs(s.populations() - ref).max()
-            print(f"force field {coll} hip/{variant}: {err:.3e}")
+            print(f"force field {len(kw['shape'])}D {kw['collision']} hip/{variant}: {err:.3e}")
             assert err < 3e-6
 
 

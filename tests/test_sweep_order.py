@@ -2,7 +2,7 @@
 
 pylbm_hip_run, pylbm_hip_run_store and pylbm_hip_adjoint_run alternate the order in which
 workgroups take their work items (every launch for the AA pattern, every step for states
-resident in the Infinity Cache).  Every node is computed from the same inputs whatever the
+resident in the Infinity Cache and for the geometry kernel).  Every node is computed from the same inputs whatever the
 order, so a loop must equal the same launches made one at a time in the forward order,
 bit for bit: this checks that the reversed order still visits every work item exactly
 once, including packed short rows and grids whose workgroup count is not a multiple of 8
@@ -42,7 +42,23 @@ CASES = {
                                           periodic=(True, True, True))),
     "aa-resident-3d": ("fused-aa-bf16", dict(shape=(96, 40, 24), tau=0.7, collision="trt",
                                              u_lid=0.04)),
+    # geometry kernel (alternates at every size): 4 nodes per lane at 16-bit, 2 segments
+    "geometry-fp16": ("fused-fp16", dict(shape=(2048, 300), tau=0.6, collision="trt",
+                                         periodic=(False, False), outlet=True,
+                                         inlet=np.full(300, 0.03))),
+    "geometry-3d": ("gather", dict(shape=(40, 24, 16), tau=0.7, collision="bgk",
+                                   periodic=(True, False, True), force=(1e-5, 0.0, 0.0))),
 }
+
+
+def _spheres(shape):
+    """A solid field of a few balls (discs in 2D) away from the walls."""
+    grids = np.meshgrid(*[np.arange(n) for n in reversed(shape)], indexing="ij")
+    solid = np.zeros(tuple(reversed(shape)), bool)
+    for k in range(3):
+        c = [(0.3 + 0.2 * k) * n for n in reversed(shape)]
+        solid |= sum((g - ci) ** 2 for g, ci in zip(grids, c)) <= (0.08 * min(shape)) ** 2
+    return solid
 
 
 @pytest.mark.parametrize("name", list(CASES))
@@ -52,6 +68,8 @@ def test_run_loop_equals_single_launches(name):
     ashape = tuple(reversed(kw["shape"]))
     u0 = 0.02 * rng.standard_normal((len(ashape),) + ashape)
     n = 6
+    if name.startswith("geometry"):
+        kw = dict(kw, solid=_spheres(kw["shape"]))
     a, b = (B.make("hip", variant, Problem(**kw), init="none") for _ in range(2))
     for s in (a, b):
         s.init_equilibrium(None, u0)

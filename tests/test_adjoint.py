@@ -65,6 +65,17 @@ def test_torch_restatement_gradient_matches_finite_differences():
     assert abs(float((gF * V).sum()) - float(fdF)) < 1e-6 * abs(float(fdF))
 
 
+def test_lam16_scale_is_a_power_of_two_near_the_target():
+    from pylbm_amd.adjoint import lam16_scale
+
+    for g in (1e-9, 3e-4, 0.03, 1.0, 7.5e3, 1e12):
+        s = lam16_scale(g)
+        assert s == 2.0 ** round(np.log2(s))
+        assert 2.0 ** 7 <= s * g <= 2.0 ** 9
+    assert lam16_scale(0.0) == 1.0 and lam16_scale(float("nan")) == 1.0
+    assert lam16_scale(float("inf")) == 1.0
+
+
 @pytest.mark.parametrize("coll", ["bgk", "trt"])
 def test_tau_rate_derivatives(coll):
     p = Problem((4, 4), tau=0.8, collision=coll)
@@ -255,6 +266,31 @@ def test_channel_gradients_vs_float64_autograd_and_fd():
     assert eF < 1e-4 and et < 1e-4
     assert abs(gt_hip - fd_t) < 1e-4 * abs(fd_t)
     assert abs(dd - fd_F) < 1e-4 * abs(fd_F)
+
+
+@pytest.mark.gpu
+def test_channel_gradients_with_fp16_adjoint_states():
+    """fp16 adjoint states on the force-field channel (128x64, T = 200): the rollout scales
+    the terminal gradient by a power of two for fp16's range and divides dL/dF and dL/dtau
+    by it again; both stay within the fp16 rounding of the fp32-state gradients."""
+    from pylbm_amd.adjoint import RolloutConfig, lam16_scale, rollout, velocity_loss
+
+    nx, ny, T = 128, 64, 200
+    ff, ut = _channel_setup(nx, ny)
+    kw = dict(shape=(nx, ny), tau=0.8, periodic=(True, False), collision="bgk")
+    p = Problem(**kw, force_field=ff)
+    got = {}
+    for ls in ("f32", "f16"):
+        cfg = RolloutConfig(p, T, checkpoint=50, lam_storage=ls)
+        F = _dev(ff).float().requires_grad_(True)
+        tau = torch.tensor(0.8, dtype=torch.float64, device="cuda", requires_grad=True)
+        velocity_loss(rollout(cfg, F, tau), _dev(ut)).backward()
+        got[ls] = (F.grad.double().cpu().numpy(), float(tau.grad))
+    eF = np.abs(got["f16"][0] - got["f32"][0]).max() / np.abs(got["f32"][0]).max()
+    et = abs(got["f16"][1] - got["f32"][1]) / abs(got["f32"][1])
+    print(f"channel fp16 adjoint states: dF rel err {eF:.2e}, dtau rel err {et:.2e} "
+          f"(scale {lam16_scale(1e-2):g} for |dL/du| = 1e-2)")
+    assert eF < 2e-3 and et < 2e-3  # measured 1.2e-4 and 2.3e-4
 
 
 @pytest.mark.gpu

@@ -397,8 +397,8 @@ int pylbm_hip_adjoint_step(const pylbm_params* p, const void* f_t, const void* l
   KArgs a;
   if (int e = make_kargs(p, a, "pylbm_hip_adjoint_step")) return e;
   return hip_fail(launch_adjoint_step(p->lattice, a, p->collision, p->storage, p->lam_storage,
-                                      p->force_field != nullptr, f_t, lam_in, lam_out, dF,
-                                      dtau_part, (hipStream_t)stream),
+                                      PYLBM_AB, p->force_field != nullptr, f_t, lam_in, lam_out,
+                                      dF, dtau_part, (hipStream_t)stream),
                   "pylbm_hip_adjoint_step");
 }
 
@@ -415,21 +415,30 @@ int pylbm_hip_adjoint_run(const pylbm_params* p, const void* const* states, int6
   if (int e = make_kargs(p, a, "pylbm_hip_adjoint_run")) return e;
   KArgs ar = a;
   ar.reverse = 1;
+  for (int64_t k = 0; k < nsteps; ++k)
+    if (!states[k]) return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_run: states[%lld] is NULL",
+                                (long long)k);
+  const int dims = p->lattice == PYLBM_D2Q9 ? 2 : 3;
+  // A box periodic on every axis runs its steps in pairs in place in lam_a (the AA pattern
+  // of the adjoint, k_adjoint): one adjoint-state array instead of two, which a state
+  // resident in the Infinity Cache keeps there.  An odd last step goes lam_a -> lam_b.
+  const bool aa = a.per == (1 << dims) - 1;
   void* lin = lam_a;
   void* lout = lam_b;
   for (int64_t k = nsteps - 1; k >= 0; --k) {
-    if (!states[k]) return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_run: states[%lld] is NULL",
-                                (long long)k);
     // the sweep direction alternates every step for resident states (remap_block)
     const bool rev = ((nsteps - 1 - k) & 1) && a.resident;
-    if (int e = launch_adjoint_step(p->lattice, rev ? ar : a, p->collision,
-                                    p->storage, p->lam_storage,
-                                    p->force_field != nullptr, states[k], lin, lout, dF,
-                                    dtau_part, (hipStream_t)stream))
+    const bool pair = aa && (nsteps - 1 - k) < nsteps - nsteps % 2;
+    const int pat = !pair ? PYLBM_AB : (((nsteps - 1 - k) & 1) ? PYLBM_AA_ODD : PYLBM_AA_EVEN);
+    if (int e = launch_adjoint_step(p->lattice, rev ? ar : a, p->collision, p->storage,
+                                    p->lam_storage, pat, p->force_field != nullptr, states[k],
+                                    lin, pair ? lin : lout, dF, dtau_part, (hipStream_t)stream))
       return hip_fail(e, "pylbm_hip_adjoint_run");
-    void* t = lin;
-    lin = lout;
-    lout = t;
+    if (!pair) {
+      void* t = lin;
+      lin = lout;
+      lout = t;
+    }
   }
   *which = (lin == lam_a) ? 0 : 1;
   return 0;

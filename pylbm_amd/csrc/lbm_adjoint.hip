@@ -142,11 +142,20 @@ __device__ __forceinline__ float wave_sum_f(float v) {
 
 // LT is the element type of the adjoint states lam_in / lam_out: float, or _Float16 (values
 // rounded to nearest fp16 on store; every operation on them is fp32).
-template <class L, int COLL, int ST, int FM, int ADDR, class LT, int SP>
+//
+// PAT selects the layout of the adjoint states (the AA pattern run backwards, for boxes
+// periodic on every axis, where no link meets a wall):
+//   PYLBM_AB       lam_in natural -> lam_out natural, two buffers (push to (i, x - c_i));
+//   PYLBM_AA_EVEN  in place: reads the own natural slots (i, x), writes mu_i to the own
+//                  slot opp(i) (the push along c_i is left to the next step);
+//   PYLBM_AA_ODD   in place: reads lam_i(x) from (opp(i), x + c_i), where the even step
+//                  left it, and pushes mu_i to (i, x - c_i), natural again.
+// In both AA steps a node writes exactly the slots it read, so the update in place has no
+// race; a pair of them equals two two-buffer steps bit for bit.
+template <class L, int COLL, int ST, int FM, int ADDR, class LT, int SP, int PAT>
 __global__ void __launch_bounds__(kBlock)
-k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f,
-          const LT* __restrict__ lam_in, LT* __restrict__ lam_out,
-          float* __restrict__ dF, double* __restrict__ part) {
+k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f, const LT* lam_in,
+          LT* lam_out, float* __restrict__ dF, double* __restrict__ part) {
   using T = typename Codec<ST>::T;
   constexpr int Q = L::Q;
   constexpr uint32_t TS = sizeof(T);
@@ -168,8 +177,18 @@ k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f,
     T t[Q];
     float lid[Q];
     gather_issue<L, ST, PYLBM_AB, ADDR>(a, l, fb, N * TS, t, lid);
+    if constexpr (PAT == PYLBM_AA_ODD) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) lam[i] = (float)ld_pl<LT, ADDR>(lib, lplane, i, 0, q0);
+      for (int i = 0; i < Q; ++i) {
+        const int cx = L::c(i, 0);
+        const RowLink r = row_link<L>(a, l.y, l.z, i, +1);
+        lam[i] = (float)ld_pl<LT, ADDR>(lib, lplane, L::opp(i), r.delta * (ptrdiff_t)LS,
+                                        cx > 0 ? qp : (cx < 0 ? qm : q0));
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < Q; ++i) lam[i] = (float)ld_pl<LT, ADDR>(lib, lplane, i, 0, q0);
+    }
     constexpr bool FF = FM == 2;
     float F[3] = {a.fx, a.fy, a.fz};
     const char* ffb = reinterpret_cast<const char*>(a.ff);
@@ -188,7 +207,10 @@ k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f,
     // slot opp(i) on a wall link.  Waves with no lane on an x wall (every wave of a box
     // periodic in x) take a loop without per-lane branches; the per-lane wall choice costs
     // each direction a divergent branch and its exec-mask bookkeeping on the scalar unit.
-    if (__ballot(l.wall_lo || l.wall_hi) == 0) {
+    if constexpr (PAT == PYLBM_AA_EVEN) {
+#pragma unroll
+      for (int i = 0; i < Q; ++i) st_pl<SP, ADDR>(lob, lplane, L::opp(i), 0, q0, (LT)mu[i]);
+    } else if (__ballot(l.wall_lo || l.wall_hi) == 0) {
       // a y/z wall (uniform) selects the own slot opp(i) by value, without a branch
 #pragma unroll
       for (int i = 0; i < Q; ++i) {
@@ -322,9 +344,9 @@ k_adjoint_init(int N_, const double* __restrict__ rho0, const double* __restrict
   if (drho) drho[n] = gr;
 }
 
-template <class L, int COLL, int ST, class LT>
-int adj_step_t(const KArgs& a, bool field, const void* f, const void* lin, void* lout,
-               float* dF, double* part, hipStream_t s) {
+template <class L, int COLL, int ST, class LT, int PAT>
+int adj_step_tp(const KArgs& a, bool field, const void* f, const void* lin, void* lout,
+                float* dF, double* part, hipStream_t s) {
   using T = typename Codec<ST>::T;
   const dim3 grid((unsigned)a.nblk), block(kBlock);
   const bool uniform = a.fx != 0.0f || a.fy != 0.0f || a.fz != 0.0f;
@@ -338,52 +360,65 @@ int adj_step_t(const KArgs& a, bool field, const void* f, const void* lin, void*
   // Cache (the next adjoint step re-reads them), non-temporal otherwise
   if (small && a.resident) {
     if (field)
-      k_adjoint<L, COLL, ST, 2, 1, LT, 0><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
+      k_adjoint<L, COLL, ST, 2, 1, LT, 0, PAT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
     else if (uniform)
-      k_adjoint<L, COLL, ST, 1, 1, LT, 0><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
+      k_adjoint<L, COLL, ST, 1, 1, LT, 0, PAT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
     else
-      k_adjoint<L, COLL, ST, 0, 1, LT, 0><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
+      k_adjoint<L, COLL, ST, 0, 1, LT, 0, PAT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
   } else if (small) {
     if (field)
-      k_adjoint<L, COLL, ST, 2, 1, LT, 2><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
+      k_adjoint<L, COLL, ST, 2, 1, LT, 2, PAT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
     else if (uniform)
-      k_adjoint<L, COLL, ST, 1, 1, LT, 2><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
+      k_adjoint<L, COLL, ST, 1, 1, LT, 2, PAT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
     else
-      k_adjoint<L, COLL, ST, 0, 1, LT, 2><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
+      k_adjoint<L, COLL, ST, 0, 1, LT, 2, PAT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
   } else {
     if (field)
-      k_adjoint<L, COLL, ST, 2, 0, LT, 2><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
+      k_adjoint<L, COLL, ST, 2, 0, LT, 2, PAT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
     else if (uniform)
-      k_adjoint<L, COLL, ST, 1, 0, LT, 2><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
+      k_adjoint<L, COLL, ST, 1, 0, LT, 2, PAT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
     else
-      k_adjoint<L, COLL, ST, 0, 0, LT, 2><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
+      k_adjoint<L, COLL, ST, 0, 0, LT, 2, PAT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
   }
   return (int)hipGetLastError();
 }
 
-template <class L, int COLL, class LT>
-int adj_step_s(const KArgs& a, int storage, bool field, const void* f, const void* lin,
+template <class L, int COLL, int ST, class LT>
+int adj_step_t(const KArgs& a, int pattern, bool field, const void* f, const void* lin,
                void* lout, float* dF, double* part, hipStream_t s) {
+  switch (pattern) {
+    case PYLBM_AB: return adj_step_tp<L, COLL, ST, LT, PYLBM_AB>(a, field, f, lin, lout, dF, part, s);
+    case PYLBM_AA_EVEN:
+      return adj_step_tp<L, COLL, ST, LT, PYLBM_AA_EVEN>(a, field, f, lin, lout, dF, part, s);
+    case PYLBM_AA_ODD:
+      return adj_step_tp<L, COLL, ST, LT, PYLBM_AA_ODD>(a, field, f, lin, lout, dF, part, s);
+  }
+  return PYLBM_EINVAL;
+}
+
+template <class L, int COLL, class LT>
+int adj_step_s(const KArgs& a, int storage, int pattern, bool field, const void* f,
+               const void* lin, void* lout, float* dF, double* part, hipStream_t s) {
   switch (storage) {
     case PYLBM_F32:
-      return adj_step_t<L, COLL, PYLBM_F32, LT>(a, field, f, lin, lout, dF, part, s);
+      return adj_step_t<L, COLL, PYLBM_F32, LT>(a, pattern, field, f, lin, lout, dF, part, s);
     case PYLBM_F16:
-      return adj_step_t<L, COLL, PYLBM_F16, LT>(a, field, f, lin, lout, dF, part, s);
+      return adj_step_t<L, COLL, PYLBM_F16, LT>(a, pattern, field, f, lin, lout, dF, part, s);
     case PYLBM_BF16:
-      return adj_step_t<L, COLL, PYLBM_BF16, LT>(a, field, f, lin, lout, dF, part, s);
+      return adj_step_t<L, COLL, PYLBM_BF16, LT>(a, pattern, field, f, lin, lout, dF, part, s);
     case PYLBM_F32S:
-      return adj_step_t<L, COLL, PYLBM_F32S, LT>(a, field, f, lin, lout, dF, part, s);
+      return adj_step_t<L, COLL, PYLBM_F32S, LT>(a, pattern, field, f, lin, lout, dF, part, s);
   }
   return PYLBM_EINVAL;
 }
 
 template <class L, class LT>
-int adj_step_l(const KArgs& a, int coll, int storage, bool field, const void* f,
+int adj_step_l(const KArgs& a, int coll, int storage, int pattern, bool field, const void* f,
                const void* lin, void* lout, float* dF, double* part, hipStream_t s) {
   if (coll == PYLBM_BGK)
-    return adj_step_s<L, PYLBM_BGK, LT>(a, storage, field, f, lin, lout, dF, part, s);
+    return adj_step_s<L, PYLBM_BGK, LT>(a, storage, pattern, field, f, lin, lout, dF, part, s);
   if (coll == PYLBM_TRT)
-    return adj_step_s<L, PYLBM_TRT, LT>(a, storage, field, f, lin, lout, dF, part, s);
+    return adj_step_s<L, PYLBM_TRT, LT>(a, storage, pattern, field, f, lin, lout, dF, part, s);
   return PYLBM_ENOTSUP;
 }
 
@@ -412,12 +447,12 @@ int seed_l(const KArgs& a, int storage, bool field, const void* f, const float* 
 }
 
 template <class L>
-int adj_step_lt(const KArgs& a, int coll, int storage, int lam_storage, bool field,
+int adj_step_lt(const KArgs& a, int coll, int storage, int lam_storage, int pattern, bool field,
                 const void* f, const void* lin, void* lout, float* dF, double* part,
                 hipStream_t s) {
   if (lam_storage == PYLBM_F16)
-    return adj_step_l<L, _Float16>(a, coll, storage, field, f, lin, lout, dF, part, s);
-  return adj_step_l<L, float>(a, coll, storage, field, f, lin, lout, dF, part, s);
+    return adj_step_l<L, _Float16>(a, coll, storage, pattern, field, f, lin, lout, dF, part, s);
+  return adj_step_l<L, float>(a, coll, storage, pattern, field, f, lin, lout, dF, part, s);
 }
 
 template <class L>
@@ -443,11 +478,13 @@ void init_lt(int lam_storage, dim3 grid, int N, const double* rho0, const double
 int adjoint_partials(const KArgs& a) { return 2 * a.nblk; }
 
 int launch_adjoint_step(int lattice, const KArgs& a, int coll, int storage, int lam_storage,
-                        bool field, const void* f, const void* lin, void* lout, float* dF,
-                        double* part, hipStream_t s) {
+                        int pattern, bool field, const void* f, const void* lin, void* lout,
+                        float* dF, double* part, hipStream_t s) {
   if (lattice == PYLBM_D2Q9)
-    return adj_step_lt<D2Q9>(a, coll, storage, lam_storage, field, f, lin, lout, dF, part, s);
-  return adj_step_lt<D3Q19>(a, coll, storage, lam_storage, field, f, lin, lout, dF, part, s);
+    return adj_step_lt<D2Q9>(a, coll, storage, lam_storage, pattern, field, f, lin, lout, dF,
+                             part, s);
+  return adj_step_lt<D3Q19>(a, coll, storage, lam_storage, pattern, field, f, lin, lout, dF,
+                            part, s);
 }
 
 int launch_adjoint_seed(int lattice, const KArgs& a, int storage, int lam_storage, bool field,

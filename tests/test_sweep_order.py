@@ -85,12 +85,16 @@ def test_run_loop_equals_single_launches(name):
     assert d == 0.0
 
 
-@pytest.mark.parametrize("shape,periodic", [((24, 16), (True, False)),
-                                            ((32, 32, 32), (True, True, True))])
-def test_adjoint_run_equals_single_adjoint_steps(shape, periodic):
-    """pylbm_hip_adjoint_run (alternating on these resident states) vs single adjoint
-    steps: the same adjoint state and force gradient bit for bit; the dL/ds partial sums
-    land in other per-workgroup slots, so their total agrees to float64 rounding."""
+@pytest.mark.parametrize("shape,periodic,n", [((24, 16), (True, False), 5),
+                                              ((32, 32, 32), (True, True, True), 5),
+                                              ((32, 32, 32), (True, True, True), 6),
+                                              ((300, 40), (True, True), 4)])
+def test_adjoint_run_equals_single_adjoint_steps(shape, periodic, n):
+    """pylbm_hip_adjoint_run (alternating on these resident states; on boxes periodic on
+    every axis the in-place AA pattern in pairs, an odd last step two-buffer) vs single
+    two-buffer adjoint steps: the same adjoint state and force gradient bit for bit; the
+    dL/ds partial sums land in other per-workgroup slots, so their total agrees to float64
+    rounding."""
     from pylbm_amd.adjoint import RolloutConfig, _Engine, _ptr_array
 
     rng = np.random.default_rng(8)
@@ -102,7 +106,6 @@ def test_adjoint_run_equals_single_adjoint_steps(shape, periodic):
     q, N = p.lattice.q, p.n_nodes
     ffd = torch.as_tensor(ff, device="cuda").reshape(d, -1)
     eng = _Engine(RolloutConfig(p, 1, storage="f32s"), ffd, 0.7)
-    n = 5
     states = []
     f = eng.state()
     u0 = torch.as_tensor(0.02 * rng.standard_normal((d,) + ashape), device="cuda")

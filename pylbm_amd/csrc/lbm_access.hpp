@@ -74,7 +74,8 @@ __device__ __forceinline__ void plane_ref(const char* A, size_t plane, int slot,
   }
 }
 
-template <class T, int ADDR>
+// LA: cache policy bits of the load (0 default; 2 non-temporal)
+template <class T, int ADDR, int LA = 0>
 __device__ __forceinline__ T ld_pl(const char* A, size_t plane, int slot, ptrdiff_t delta,
                                    uint32_t off) {
   if constexpr (ADDR >= 1) {
@@ -82,9 +83,9 @@ __device__ __forceinline__ T ld_pl(const char* A, size_t plane, int slot, ptrdif
     uint32_t so;
     plane_ref<ADDR>(A, plane, slot, delta, base, so);
     if constexpr (sizeof(T) == 4)
-      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rsrc_at(base), off, so, 0));
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rsrc_at(base), off, so, LA));
     else
-      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(rsrc_at(base), off, so, 0));
+      return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(rsrc_at(base), off, so, LA));
   } else {
     return ld_at<T>(A + slot * plane + delta, off);
   }
@@ -214,7 +215,7 @@ __device__ __forceinline__ RowLink row_link(const KArgs& a, int y, int z, int i,
 // gather = gather_issue (every load, back to back) + gather_finish (x-wall bounce-back in
 // one divergent block, decode); split so that a lane serving several nodes issues the
 // loads of all of them before it waits for any.
-template <class L, int ST, int PAT, int ADDR>
+template <class L, int ST, int PAT, int ADDR, int LA = 0>
 __device__ __forceinline__ void gather_issue(const KArgs& a, const Lane& l, const char* Ab,
                                              size_t plane, typename Codec<ST>::T (&t)[L::Q],
                                              float (&lid)[L::Q]) {
@@ -229,12 +230,12 @@ __device__ __forceinline__ void gather_issue(const KArgs& a, const Lane& l, cons
     lid[i] = 0.0f;
     if (PAT == PYLBM_AA_EVEN || i == 0) {
       const int slot = (PAT == PYLBM_AA_ODD) ? L::opp(i) : i;
-      t[i] = ld_pl<T, ADDR>(Ab, plane, slot, 0, l.o0);
+      t[i] = ld_pl<T, ADDR, LA>(Ab, plane, slot, 0, l.o0);
       continue;
     }
     const RowLink r = row_link<L>(a, l.y, l.z, i, -1);
     const uint32_t offx = cx > 0 ? l.om : (cx < 0 ? l.op : l.o0);
-    t[i] = ld_pl<T, ADDR>(Ab, plane, r.wall ? own_bb : far,
+    t[i] = ld_pl<T, ADDR, LA>(Ab, plane, r.wall ? own_bb : far,
                            r.wall ? 0 : r.delta * (ptrdiff_t)TS, r.wall ? l.o0 : offx);
     if (cx != 0) lid[i] = (r.wall && r.top) ? (float)(6.0 * L::w(i) * cx) * a.ulid : 0.0f;
   }

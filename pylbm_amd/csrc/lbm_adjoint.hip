@@ -176,7 +176,9 @@ k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f, const LT* 
     float h[Q], lam[Q];
     T t[Q];
     float lid[Q];
-    gather_issue<L, ST, PYLBM_AB, ADDR>(a, l, fb, N * TS, t, lid);
+    // resident states (SP = 0): the forward state each step reads once streams past the
+    // Infinity Cache (non-temporal loads), which keeps the adjoint state there
+    gather_issue<L, ST, PYLBM_AB, ADDR, SP == 0 ? 2 : 0>(a, l, fb, N * TS, t, lid);
     if constexpr (PAT == PYLBM_AA_ODD) {
 #pragma unroll
       for (int i = 0; i < Q; ++i) {

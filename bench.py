@@ -382,7 +382,7 @@ def main() -> int:
             "frac": achieved / SPEC_BANDWIDTH_GBS,
             "traffic": traffic["bytes_per_launch"] if traffic else None,
             "kernel": "k_step<D2Q9, BGK, f32, two-buffer, no force, single descriptor, "
-                      "1 node per lane>",
+                      "1 node per lane, non-temporal stores>",
             "algorithmic_bytes_per_launch": head["bytes_per_update"] * n * n,
             "avg_launch_us": t_launch * 1e6,
             "frac_of_measured_copy": achieved / cbw,

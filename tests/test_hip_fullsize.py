@@ -133,3 +133,110 @@ def test_d3q19_512_aa_matches_two_buffer_and_conserves_mass(storage):
     print(f"512^3 TRT{storage}: AB vs AA max|df| = {d:.3e}; mass {m0:.9e} -> {m1:.9e}")
     assert d < AA_TOL[storage]
     assert abs(m1 - m0) <= 1e-6 * m0
+
+
+# ----------------------------------------------------------------- adjoint at the box size
+
+
+def test_adjoint_in_place_at_128_cubed_equals_two_buffer_steps():
+    """The 128^3 periodic box of the differentiable D3Q19 case: pylbm_hip_adjoint_run runs
+    pairs of steps in place (AA pattern) with alternating sweeps; single two-buffer adjoint
+    steps give the same adjoint state, bit for bit with fp32 adjoint states and within one
+    fp16 ulp of the field's scale with fp16 ones (measured: 0.5% of the slots differ, by at
+    most 1.00 ulp; the kernels fuse the rounding to fp16 differently, v_fma_mixlo_f16 vs
+    an fp32 add then a conversion), far below what a stale value would differ by."""
+    import ctypes
+
+    import torch
+
+    from pylbm_amd import _lib
+    from pylbm_amd.adjoint import RolloutConfig, _Engine, _ptr_array
+    from pylbm_amd.fields import divergence_free_field
+
+    n, steps = 128, 4
+    p = Problem((n,) * 3, tau=0.7, periodic=(True,) * 3)
+    u0 = torch.as_tensor(divergence_free_field(n, 8, 0.02, 1), device="cuda")
+    for ls, dt in (("f32", torch.float32), ("f16", torch.float16)):
+        eng = _Engine(RolloutConfig(p, 1, storage="f16", lam_storage=ls), None, 0.7)
+        states, f = [], eng.state()
+        eng.init(None, u0, f)
+        for _ in range(steps):
+            states.append(f)
+            g = eng.state()
+            eng.step(f, g)
+            f = g
+        q, N = p.lattice.q, p.n_nodes
+        lam0 = (0.01 * torch.randn(q * N, device="cuda", generator=torch.Generator(
+            device="cuda").manual_seed(3))).to(dt)
+        nw = int(_lib.load().pylbm_hip_adjoint_workspace(eng.ref))
+        out = []
+        for mode in ("run", "steps"):
+            lam = [lam0.clone(), torch.empty_like(lam0)]
+            part = torch.zeros(nw, dtype=torch.float64, device="cuda")
+            if mode == "run":
+                which = ctypes.c_int(0)
+                _lib.call("pylbm_hip_adjoint_run", eng.ref, _ptr_array(states), steps,
+                          lam[0].data_ptr(), lam[1].data_ptr(), None, part.data_ptr(),
+                          ctypes.byref(which), None)
+                res = lam[which.value]
+            else:
+                cur = 0
+                for k in range(steps - 1, -1, -1):
+                    _lib.call("pylbm_hip_adjoint_step", eng.ref, states[k].data_ptr(),
+                              lam[cur].data_ptr(), lam[1 - cur].data_ptr(), None,
+                              part.data_ptr(), None)
+                    cur = 1 - cur
+                res = lam[cur]
+            torch.cuda.synchronize()
+            out.append(res.clone())
+        if ls == "f32":
+            print(f"128^3 adjoint, fp32 states: in place vs two-buffer equal = "
+                  f"{bool(torch.equal(out[0], out[1]))}")
+            assert torch.equal(out[0], out[1])
+        else:
+            # the kernels may fuse the last add with the rounding to fp16 differently
+            # (v_fma_mixlo_f16 rounds once, an fp32 add then a conversion twice), so a few
+            # slots may differ by one fp16 ulp; a race would show whole wrong values
+            a16, b16 = out[0].float(), out[1].float()
+            scale = float(b16.abs().max())
+            top_ulp = 2.0 ** (np.floor(np.log2(scale)) - 10)  # fp16 ulp at the field's scale
+            dmax = float((a16 - b16).abs().max())
+            ndiff = int((out[0] != out[1]).sum())
+            # the state one step earlier differs from this one by far more (a stale value)
+            step_change = float((lam0.float() - b16).abs().max())
+            print(f"128^3 adjoint, fp16 states: {ndiff} of {out[0].numel()} slots differ; "
+                  f"max |diff| {dmax:.3e} = {dmax / top_ulp:.2f} fp16 ulp at the field's scale "
+                  f"{scale:.3e}; the four steps changed it by up to {step_change:.3e}")
+            assert dmax <= 2 * top_ulp
+        del states, out, lam
+        torch.cuda.empty_cache()
+
+
+def test_box3d_rollout_gradient_fp16_states_at_full_size():
+    """The differentiable D3Q19 case at its full size (128^3, T = 500, checkpoints every 50,
+    fp16-shifted forward storage): the fp16-state gradient dL/du0 stays within fp16
+    rounding of the fp32-state gradient, and both rollouts give the same loss."""
+    import torch
+
+    from pylbm_amd.adjoint import RolloutConfig, rollout, velocity_loss
+    from pylbm_amd.fields import divergence_free_field
+
+    n, T = 128, 500
+    p = Problem((n,) * 3, tau=0.7, periodic=(True,) * 3)
+    u0 = torch.as_tensor(divergence_free_field(n, 8, 0.02, 1), device="cuda")
+    uo = torch.as_tensor(divergence_free_field(n, 8, 0.02, 2), device="cuda")
+    res = {}
+    for ls in ("f32", "f16"):
+        cfg = RolloutConfig(p, T, checkpoint=50, storage="f16", lam_storage=ls)
+        U = u0.clone().requires_grad_(True)
+        L = velocity_loss(rollout(cfg, u0=U), uo)
+        L.backward()
+        res[ls] = (float(L), U.grad.clone())
+        del U, L
+        torch.cuda.empty_cache()
+    g32, g16 = res["f32"][1], res["f16"][1]
+    err = float((g16 - g32).abs().max() / g32.abs().max())
+    print(f"128^3 T={T}: loss {res['f32'][0]:.9e} / {res['f16'][0]:.9e}; "
+          f"du0 fp16-state vs fp32-state rel err {err:.2e}")
+    assert res["f32"][0] == res["f16"][0]  # same forward rollout
+    assert err < 3e-3  # fp16 rounding of the adjoint states (3e-4 at 32^3, T = 100)

@@ -65,6 +65,17 @@ def test_torch_restatement_gradient_matches_finite_differences():
     assert abs(float((gF * V).sum()) - float(fdF)) < 1e-6 * abs(float(fdF))
 
 
+def test_rollout_config_checks_adjoint_storage():
+    from pylbm_amd.adjoint import RolloutConfig
+
+    p = Problem((8, 8), tau=0.8)
+    assert RolloutConfig(p, 4, lam_storage="f16").lam_storage == "f16"
+    with pytest.raises(ValueError, match="lam_storage"):
+        RolloutConfig(p, 4, lam_storage="bf16")
+    with pytest.raises(ValueError, match="BGK and TRT"):
+        RolloutConfig(Problem((8, 8), tau=0.8, collision="mrt"), 4)
+
+
 def test_lam16_scale_is_a_power_of_two_near_the_target():
     from pylbm_amd.adjoint import lam16_scale
 

@@ -2,8 +2,9 @@
 
 pylbm_hip_run, pylbm_hip_run_store and pylbm_hip_adjoint_run alternate the order in which
 workgroups take their work items (every launch for the AA pattern, every step for states
-resident in the Infinity Cache and for the geometry kernel).  Every node is computed from the same inputs whatever the
-order, so a loop must equal the same launches made one at a time in the forward order,
+resident in the Infinity Cache and for the geometry kernel).  Every node is computed from
+the same inputs whatever the order, so a loop must equal the same launches made one at a
+time in the forward order,
 bit for bit: this checks that the reversed order still visits every work item exactly
 once, including packed short rows and grids whose workgroup count is not a multiple of 8
 (where the whole order is reversed instead of each XCD's sequence).

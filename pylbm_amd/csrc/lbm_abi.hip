@@ -418,11 +418,10 @@ int pylbm_hip_adjoint_run(const pylbm_params* p, const void* const* states, int6
   for (int64_t k = 0; k < nsteps; ++k)
     if (!states[k]) return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_run: states[%lld] is NULL",
                                 (long long)k);
-  const int dims = p->lattice == PYLBM_D2Q9 ? 2 : 3;
-  // A box periodic on every axis runs its steps in pairs in place in lam_a (the AA pattern
-  // of the adjoint, k_adjoint): one adjoint-state array instead of two, which a state
-  // resident in the Infinity Cache keeps there.  An odd last step goes lam_a -> lam_b.
-  const bool aa = a.per == (1 << dims) - 1;
+  // The steps run in pairs in place in lam_a (the AA pattern of the adjoint, k_adjoint):
+  // one adjoint-state array instead of two, which a state resident in the Infinity Cache
+  // keeps there.  An odd last step goes lam_a -> lam_b.
+  const bool aa = true;
   void* lin = lam_a;
   void* lout = lam_b;
   for (int64_t k = nsteps - 1; k >= 0; --k) {

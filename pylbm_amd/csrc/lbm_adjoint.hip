@@ -43,6 +43,9 @@ __device__ __forceinline__ void adjoint_node(const float (&h)[L::Q], const float
                                              const float (&lam)[L::Q], float sp, float sm,
                                              float (&mu)[L::Q], float dF[3], float& dsp,
                                              float& dsm) {
+  // multiply-adds fuse within a statement only, so the two-buffer and the in-place (AA)
+  // adjoint kernels round every node identically whatever their store pattern
+#pragma clang fp contract(on)
   constexpr int Q = L::Q, D = L::D;
   float drho = 0.f, j[3] = {0.f, 0.f, 0.f};
 #pragma unroll
@@ -143,13 +146,15 @@ __device__ __forceinline__ float wave_sum_f(float v) {
 // LT is the element type of the adjoint states lam_in / lam_out: float, or _Float16 (values
 // rounded to nearest fp16 on store; every operation on them is fp32).
 //
-// PAT selects the layout of the adjoint states (the AA pattern run backwards, for boxes
-// periodic on every axis, where no link meets a wall):
-//   PYLBM_AB       lam_in natural -> lam_out natural, two buffers (push to (i, x - c_i));
+// PAT selects the layout of the adjoint states (the AA pattern run backwards):
+//   PYLBM_AB       lam_in natural -> lam_out natural, two buffers (push to (i, x - c_i),
+//                  or to the own slot opp(i) on a wall link);
 //   PYLBM_AA_EVEN  in place: reads the own natural slots (i, x), writes mu_i to the own
 //                  slot opp(i) (the push along c_i is left to the next step);
 //   PYLBM_AA_ODD   in place: reads lam_i(x) from (opp(i), x + c_i), where the even step
-//                  left it, and pushes mu_i to (i, x - c_i), natural again.
+//                  left it (from the own slot i when x + c_i is behind a wall: the
+//                  bounced-back mu_opp(i)(x)), and pushes mu_i as PYLBM_AB does, which
+//                  leaves the state natural again.
 // In both AA steps a node writes exactly the slots it read, so the update in place has no
 // race; a pair of them equals two two-buffer steps bit for bit.
 template <class L, int COLL, int ST, int FM, int ADDR, class LT, int SP, int PAT>
@@ -180,12 +185,27 @@ k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f, const LT* 
     // Infinity Cache (non-temporal loads), which keeps the adjoint state there
     gather_issue<L, ST, PYLBM_AB, ADDR, SP == 0 ? 2 : 0>(a, l, fb, N * TS, t, lid);
     if constexpr (PAT == PYLBM_AA_ODD) {
+      // lam_i(x) is mu_i(x + c_i), which the even step left in (opp(i), x + c_i); when
+      // x + c_i lies behind a wall it is the bounced-back mu_opp(i)(x), left in the own
+      // slot i (y/z walls are uniform and selected by value; x-wall lanes re-read their
+      // own slots afterwards, in one divergent block, as the gather does)
 #pragma unroll
       for (int i = 0; i < Q; ++i) {
         const int cx = L::c(i, 0);
         const RowLink r = row_link<L>(a, l.y, l.z, i, +1);
-        lam[i] = (float)ld_pl<LT, ADDR>(lib, lplane, L::opp(i), r.delta * (ptrdiff_t)LS,
-                                        cx > 0 ? qp : (cx < 0 ? qm : q0));
+        lam[i] = (float)ld_pl<LT, ADDR>(lib, lplane, r.wall ? i : L::opp(i),
+                                        r.wall ? 0 : r.delta * (ptrdiff_t)LS,
+                                        r.wall ? q0 : (cx > 0 ? qp : (cx < 0 ? qm : q0)));
+      }
+      if (l.wall_lo || l.wall_hi) {
+        LT own[Q];
+#pragma unroll
+        for (int i = 1; i < Q; ++i)
+          if (L::c(i, 0) != 0) own[i] = ld_pl<LT, ADDR>(lib, lplane, i, 0, q0);
+#pragma unroll
+        for (int i = 1; i < Q; ++i)
+          if ((L::c(i, 0) > 0 && l.wall_hi) || (L::c(i, 0) < 0 && l.wall_lo))
+            lam[i] = (float)own[i];
       }
     } else {
 #pragma unroll

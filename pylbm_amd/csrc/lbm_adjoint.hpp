@@ -6,8 +6,7 @@
 namespace plbm {
 
 int adjoint_partials(const KArgs& a);
-// pattern: PYLBM_AB (lin -> lout), or PYLBM_AA_EVEN / PYLBM_AA_ODD in place (lin == lout;
-// boxes periodic on every axis only)
+// pattern: PYLBM_AB (lin -> lout), or PYLBM_AA_EVEN / PYLBM_AA_ODD in place (lin == lout)
 int launch_adjoint_step(int lattice, const KArgs& a, int coll, int storage, int lam_storage,
                         int pattern, bool field, const void* f, const void* lin, void* lout,
                         float* dF, double* part, hipStream_t s);

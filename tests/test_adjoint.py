@@ -65,6 +65,70 @@ def test_torch_restatement_gradient_matches_finite_differences():
     assert abs(float((gF * V).sum()) - float(fdF)) < 1e-6 * abs(float(fdF))
 
 
+@pytest.mark.parametrize("lattice,shape,periodic", [
+    ("D2Q9", (9, 6), (False, False)), ("D2Q9", (8, 5), (True, False)),
+    ("D3Q19", (7, 5, 4), (False, True, False)), ("D3Q19", (6, 5, 4), (True, True, True))])
+def test_inplace_adjoint_index_rule(lattice, shape, periodic):
+    """The layout rule of the in-place (AA) adjoint, on the CPU: an even step (own slots in,
+    mu_k to the own slot opp(k)) and an odd step (lam_i(x) from (opp(i), x + c_i), or the
+    own slot i when x + c_i is behind a wall; mu pushed as the two-buffer step pushes)
+    equal two two-buffer push steps, with any per-node map standing in for the transposed
+    collision, and each node of the odd step writes exactly the slots it reads."""
+    from pylbm_amd.lattice import LATTICES
+
+    lat = LATTICES[lattice]
+    c, opp, q, d = np.array(lat.c), np.array(lat.opp), lat.q, len(shape)
+    rng = np.random.default_rng(1)
+    M = rng.standard_normal((q, q))
+    nodes = list(np.ndindex(*shape))
+
+    def inside(x):
+        y = list(x)
+        for a in range(d):
+            if not 0 <= y[a] < shape[a]:
+                if not periodic[a]:
+                    return None
+                y[a] %= shape[a]
+        return tuple(y)
+
+    def at(k, x):
+        return (k,) + tuple(reversed(x))
+
+    def mu(lam, x):
+        return M @ lam + 0.01 * sum(x)
+
+    def push(out, x, m, touched=None):
+        for k in range(q):
+            src = inside(tuple(np.array(x) - c[k, :d]))
+            slot = at(opp[k], x) if src is None else at(k, src)
+            out[slot] = m[k]
+            if touched is not None:
+                touched.append(slot)
+
+    def ab(lam):
+        out = np.full_like(lam, np.nan)
+        for x in nodes:
+            push(out, x, mu(lam[(slice(None),) + tuple(reversed(x))], x))
+        return out
+
+    lam0 = rng.standard_normal((q,) + tuple(reversed(shape)))
+    A = lam0.copy()
+    for x in nodes:  # even
+        m = mu(lam0[(slice(None),) + tuple(reversed(x))], x)
+        for k in range(q):
+            A[at(opp[k], x)] = m[k]
+    B = A.copy()
+    for x in nodes:  # odd
+        reads, writes, lam = [], [], np.empty(q)
+        for i in range(q):
+            nb = inside(tuple(np.array(x) + c[i, :d]))
+            reads.append(at(i, x) if nb is None else at(opp[i], nb))
+            lam[i] = A[reads[-1]]
+        push(B, x, mu(lam, x), writes)
+        assert sorted(reads) == sorted(writes)
+    np.testing.assert_array_equal(B, ab(ab(lam0)))
+
+
 def test_rollout_config_checks_adjoint_storage():
     from pylbm_amd.adjoint import RolloutConfig
 

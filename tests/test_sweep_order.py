@@ -87,12 +87,16 @@ def test_run_loop_equals_single_launches(name):
 
 
 @pytest.mark.parametrize("shape,periodic,n", [((24, 16), (True, False), 5),
+                                              ((24, 16), (True, False), 6),
+                                              ((40, 28), (False, False), 4),
+                                              ((300, 40), (False, True), 4),
+                                              ((20, 12, 10), (False, True, False), 6),
                                               ((32, 32, 32), (True, True, True), 5),
                                               ((32, 32, 32), (True, True, True), 6),
                                               ((300, 40), (True, True), 4)])
 def test_adjoint_run_equals_single_adjoint_steps(shape, periodic, n):
-    """pylbm_hip_adjoint_run (alternating on these resident states; on boxes periodic on
-    every axis the in-place AA pattern in pairs, an odd last step two-buffer) vs single
+    """pylbm_hip_adjoint_run (alternating on these resident states; the in-place AA
+    pattern in pairs, an odd last step two-buffer; walls on x, y, z and none) vs single
     two-buffer adjoint steps: the same adjoint state and force gradient bit for bit; the
     dL/ds partial sums land in other per-workgroup slots, so their total agrees to float64
     rounding."""

@@ -208,8 +208,8 @@ int pylbm_hip_adjoint_step(const pylbm_params* p, const void* f_t, const void* l
 /* nsteps adjoint steps over a stored segment, in reverse: for k = nsteps-1 .. 0,
  * lam_out = adjoint of step(states[k]) applied to lam_in, ping-ponging lam_a (input on
  * entry) and lam_b; *which = 0 if the final adjoint state is in lam_a, 1 if in lam_b.
- * On a box periodic on every axis, pairs of steps run in place in lam_a (an odd last step
- * goes to lam_b); the result equals the ping-pong bit for bit.
+ * Pairs of steps run in place in lam_a (an odd last step goes to lam_b); the result equals
+ * the ping-pong of pylbm_hip_adjoint_step bit for bit.
  * dF and dtau_part accumulate as in pylbm_hip_adjoint_step. */
 int pylbm_hip_adjoint_run(const pylbm_params* p, const void* const* states, int64_t nsteps,
                           void* lam_a, void* lam_b, float* dF, double* dtau_part, int* which,

@@ -421,13 +421,12 @@ int pylbm_hip_adjoint_run(const pylbm_params* p, const void* const* states, int6
   // The steps run in pairs in place in lam_a (the AA pattern of the adjoint, k_adjoint):
   // one adjoint-state array instead of two, which a state resident in the Infinity Cache
   // keeps there.  An odd last step goes lam_a -> lam_b.
-  const bool aa = true;
   void* lin = lam_a;
   void* lout = lam_b;
   for (int64_t k = nsteps - 1; k >= 0; --k) {
     // the sweep direction alternates every step for resident states (remap_block)
     const bool rev = ((nsteps - 1 - k) & 1) && a.resident;
-    const bool pair = aa && (nsteps - 1 - k) < nsteps - nsteps % 2;
+    const bool pair = (nsteps - 1 - k) < nsteps - nsteps % 2;
     const int pat = !pair ? PYLBM_AB : (((nsteps - 1 - k) & 1) ? PYLBM_AA_ODD : PYLBM_AA_EVEN);
     if (int e = launch_adjoint_step(p->lattice, rev ? ar : a, p->collision, p->storage,
                                     p->lam_storage, pat, p->force_field != nullptr, states[k],

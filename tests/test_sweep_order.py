@@ -86,31 +86,40 @@ def test_run_loop_equals_single_launches(name):
     assert d == 0.0
 
 
-@pytest.mark.parametrize("shape,periodic,n", [((24, 16), (True, False), 5),
-                                              ((24, 16), (True, False), 6),
-                                              ((40, 28), (False, False), 4),
-                                              ((300, 40), (False, True), 4),
-                                              ((20, 12, 10), (False, True, False), 6),
-                                              ((32, 32, 32), (True, True, True), 5),
-                                              ((32, 32, 32), (True, True, True), 6),
-                                              ((300, 40), (True, True), 4)])
-def test_adjoint_run_equals_single_adjoint_steps(shape, periodic, n):
+ADJ_CASES = [((24, 16), (True, False), 5, 0.0, "f32"),
+             ((24, 16), (True, False), 6, 0.0, "f32"),
+             ((40, 28), (False, False), 4, 0.0, "f32"),
+             ((300, 40), (False, True), 4, 0.0, "f32"),
+             ((20, 12, 10), (False, True, False), 6, 0.0, "f32"),
+             ((32, 32, 32), (True, True, True), 5, 0.0, "f32"),
+             ((32, 32, 32), (True, True, True), 6, 0.0, "f32"),
+             ((300, 40), (True, True), 4, 0.0, "f32"),
+             # moving lid (its constant term drops out of the adjoint) and fp16 adjoint states
+             ((40, 28), (False, False), 6, 0.05, "f32"),
+             ((40, 28), (False, False), 5, 0.05, "f16"),
+             ((300, 40), (False, True), 4, 0.0, "f16"),
+             ((20, 12, 10), (False, True, False), 6, 0.04, "f16"),
+             ((32, 32, 32), (True, True, True), 5, 0.0, "f16")]
+
+
+@pytest.mark.parametrize("shape,periodic,n,u_lid,lam_storage", ADJ_CASES)
+def test_adjoint_run_equals_single_adjoint_steps(shape, periodic, n, u_lid, lam_storage):
     """pylbm_hip_adjoint_run (alternating on these resident states; the in-place AA
-    pattern in pairs, an odd last step two-buffer; walls on x, y, z and none) vs single
-    two-buffer adjoint steps: the same adjoint state and force gradient bit for bit; the
-    dL/ds partial sums land in other per-workgroup slots, so their total agrees to float64
-    rounding."""
-    from pylbm_amd.adjoint import RolloutConfig, _Engine, _ptr_array
+    pattern in pairs, an odd last step two-buffer; walls on x, y, z and none, with and
+    without a lid; fp32 and fp16 adjoint states) vs single two-buffer adjoint steps: the
+    same adjoint state and force gradient bit for bit; the dL/ds partial sums land in other
+    per-workgroup slots, so their total agrees to float64 rounding."""
+    from pylbm_amd.adjoint import _LAM_DTYPES, RolloutConfig, _Engine, _ptr_array
 
     rng = np.random.default_rng(8)
     d = len(shape)
     ashape = tuple(reversed(shape))
     ff = (1e-5 * rng.standard_normal((d,) + ashape)).astype(np.float32)
-    p = Problem(shape, tau=0.7, periodic=periodic, collision="trt",
+    p = Problem(shape, tau=0.7, periodic=periodic, collision="trt", u_lid=u_lid,
                 force_field=ff.astype(np.float64))
     q, N = p.lattice.q, p.n_nodes
     ffd = torch.as_tensor(ff, device="cuda").reshape(d, -1)
-    eng = _Engine(RolloutConfig(p, 1, storage="f32s"), ffd, 0.7)
+    eng = _Engine(RolloutConfig(p, 1, storage="f32s", lam_storage=lam_storage), ffd, 0.7)
     states = []
     f = eng.state()
     u0 = torch.as_tensor(0.02 * rng.standard_normal((d,) + ashape), device="cuda")
@@ -121,7 +130,7 @@ def test_adjoint_run_equals_single_adjoint_steps(shape, periodic, n):
         eng.step(f, g)
         f = g
     lam0 = torch.as_tensor(0.01 * rng.standard_normal(q * N), dtype=torch.float32,
-                           device="cuda")
+                           device="cuda").to(_LAM_DTYPES[lam_storage])
     nw = int(_lib.load().pylbm_hip_adjoint_workspace(eng.ref))
     out = []
     for mode in ("run", "steps"):

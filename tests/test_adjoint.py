@@ -334,10 +334,11 @@ def test_channel_gradients_vs_float64_autograd_and_fd():
     hF = 1e-4 * np.abs(ff).max()
     fd_F = (loss64(0.8, ff + hF * V) - loss64(0.8, ff - hF * V)) / (2 * hF)
     dd = float((gF_hip * V).sum())
-    print(f"channel: L={float(L):.6e} (f64 {float(Lt):.6e}); dF rel err vs autograd {eF:.2e}; "
+    print(f"channel: L={float(L.detach()):.6e} (f64 {float(Lt.detach()):.6e}); "
+          f"dF rel err vs autograd {eF:.2e}; "
           f"dtau {gt_hip:.6e} vs autograd {float(gt):.6e} ({et:.2e}) vs FD {fd_t:.6e}; "
           f"<dF,V> {dd:.6e} vs FD {fd_F:.6e}")
-    assert abs(float(L) - float(Lt)) < 1e-5 * float(Lt)
+    assert abs(float(L.detach()) - float(Lt.detach())) < 1e-5 * float(Lt.detach())
     assert eF < 1e-4 and et < 1e-4
     assert abs(gt_hip - fd_t) < 1e-4 * abs(fd_t)
     assert abs(dd - fd_F) < 1e-4 * abs(fd_F)

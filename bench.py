@@ -11,8 +11,8 @@ that configuration (TRT, MRT, AA, fp16/bf16 shifted storage), the D3Q19 512^3 TR
 Taylor-Green runs (fp32, fp16, bf16; pull and AA), the measured copy ceiling, the roofline
 of the dominant kernel and the float64 NumPy oracle timed on the host (CPU baseline).
 
-``--profile [d2q9|d3q19|d3q19-fp16]`` runs only one step kernel (the headline by default)
-and the copy kernel (for rocprofv3).
+``--profile [d2q9|d3q19|d3q19-fp16|d3q19-aa|d3q19-aa-fp16]`` runs only one step kernel (the
+headline by default) and the copy kernel (for rocprofv3).
 """
 
 from __future__ import annotations
@@ -259,9 +259,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--quick", action="store_true", help="headline + copy ceiling only")
     ap.add_argument("--profile", nargs="?", const="d2q9", default=None,
-                    choices=["d2q9", "d3q19", "d3q19-fp16"],
+                    choices=["d2q9", "d3q19", "d3q19-fp16", "d3q19-aa", "d3q19-aa-fp16"],
                     help="only one step kernel and the copy kernel, for rocprofv3: the headline "
-                         "(d2q9, default), or D3Q19 512^3 TRT fp32 / fp16")
+                         "(d2q9, default), or D3Q19 512^3 TRT fp32 / fp16, two-buffer or AA")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint legs")
     ap.add_argument("--adjoint-only", action="store_true",
@@ -307,7 +307,8 @@ def main() -> int:
         else:
             n3 = a.n3
             p = Problem((n3,) * 3, tau=0.55, collision="trt", periodic=(True,) * 3)
-            v = "fused" if a.profile == "d3q19" else "fused-fp16"
+            v = {"d3q19": "fused", "d3q19-fp16": "fused-fp16", "d3q19-aa": "fused-aa",
+                 "d3q19-aa-fp16": "fused-aa-fp16"}[a.profile]
             s = B.make("hip", v, p, init="none")
             s.init_equilibrium(*tgv_init_device(n3))
             nbytes = 1 << 30

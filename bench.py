@@ -11,8 +11,8 @@ that configuration (TRT, MRT, AA, fp16/bf16 shifted storage), the D3Q19 512^3 TR
 Taylor-Green runs (fp32, fp16, bf16; pull and AA), the measured copy ceiling, the roofline
 of the dominant kernel and the float64 NumPy oracle timed on the host (CPU baseline).
 
-``--profile [d2q9|d3q19|d3q19-fp16|d3q19-aa|d3q19-aa-fp16]`` runs only one step kernel (the
-headline by default) and the copy kernel (for rocprofv3).
+``--profile [d2q9|d3q19|d3q19-fp16|d3q19-aa|d3q19-aa-fp16|geometry-d2q9|geometry-d3q19]`` runs
+only one step kernel (the headline by default) and the copy kernel (for rocprofv3).
 """
 
 from __future__ import annotations
@@ -259,9 +259,12 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--quick", action="store_true", help="headline + copy ceiling only")
     ap.add_argument("--profile", nargs="?", const="d2q9", default=None,
-                    choices=["d2q9", "d3q19", "d3q19-fp16", "d3q19-aa", "d3q19-aa-fp16"],
+                    choices=["d2q9", "d3q19", "d3q19-fp16", "d3q19-aa", "d3q19-aa-fp16",
+                             "geometry-d2q9", "geometry-d3q19"],
                     help="only one step kernel and the copy kernel, for rocprofv3: the headline "
-                         "(d2q9, default), or D3Q19 512^3 TRT fp32 / fp16, two-buffer or AA")
+                         "(d2q9, default), D3Q19 512^3 TRT fp32 / fp16, two-buffer or AA, or "
+                         "the geometry kernel (fp16) on the cylinder-array channel 4096^2 / "
+                         "the sphere-array box 256^3")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint legs")
     ap.add_argument("--adjoint-only", action="store_true",
@@ -304,6 +307,11 @@ def main() -> int:
             s = B.make("hip", "fused", p, init="none")
             s.init_equilibrium(rho, u)
             nbytes = 9 * n * n * 4  # the bytes one step reads (and writes)
+        elif a.profile.startswith("geometry"):
+            p = cylinder_array_channel(n) if a.profile.endswith("d2q9") else sphere_array_box(256)
+            s = B.make("hip", "fused-fp16", p, init="none")
+            s.init_equilibrium(None, None)
+            nbytes = 1 << 30
         else:
             n3 = a.n3
             p = Problem((n3,) * 3, tau=0.55, collision="trt", periodic=(True,) * 3)

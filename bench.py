@@ -5,11 +5,17 @@
 Workload of the headline number: D2Q9, fully periodic 4096 x 4096, fp32 populations, BGK
 tau = 0.6, two-buffer fused pull kernel (hip/fused), initial state the equilibrium of the
 shear wave u_x = 0.01 sin(2 pi y / 4096), u_y = 0, rho = 1 + 1e-3 U(-1, 1) per cell from a
-seeded generator.  Populations are resident in HBM before timing; ``steps`` launches are
-timed with HIP events on the launch stream.  The same line carries every other variant of
-that configuration (TRT, MRT, AA, fp16/bf16 shifted storage), the D3Q19 512^3 TRT
-Taylor-Green runs (fp32, fp16, bf16; pull and AA), the measured copy ceiling, the roofline
-of the dominant kernel and the float64 NumPy oracle timed on the host (CPU baseline).
+seeded generator.  Populations are resident in HBM before timing; after a load phase of
+``--warm-time`` seconds and ``warmup`` untimed steps, exactly ``steps`` launches are timed
+with HIP events on the launch stream (median of three such windows).  The same line carries
+every other variant of that configuration (TRT, MRT, AA, fp16/bf16 shifted storage), the
+D3Q19 512^3 TRT Taylor-Green runs (fp32, fp16, bf16; pull and AA), the geometry kernels,
+the forward+adjoint rollouts, the measured copy ceiling, the roofline of the dominant
+kernel, the north-star ratios of this run and the float64 NumPy oracle timed on the host
+(CPU baseline).  Every leg other than the headline follows the reference's protocol
+(pylbm/benchmark.py:159-180): a load phase, then ``--repeats`` windows of one calibrated
+step count, each at least ``--min-time`` seconds, median reported, with the GPU's clocks,
+power, temperatures and limiter counters recorded per leg.
 
 ``--profile [d2q9|d3q19|d3q19-fp16|d3q19-aa|d3q19-aa-fp16|geometry-d2q9|geometry-d3q19]`` runs
 only one step kernel (the headline by default) and the copy kernel (for rocprofv3).
@@ -65,23 +71,38 @@ def problem_2d(coll: str, n: int, rho, u):
     return Problem((n, n), tau=0.6, collision=coll, periodic=(True, True), **extra)
 
 
-def run_variant(variant: str, problem, init, steps: int, warmup: int, repeats: int = 3):
+def run_variant(variant: str, problem, init, proto: dict, steps: int | None = None,
+                warmup: int | None = None):
+    """One leg: make the solver, load its initial state, then either the leg protocol
+    (``time_leg``: a load phase, then ``proto['repeats']`` windows of at least
+    ``proto['min_time']`` seconds) or, for the headline, exactly ``steps`` timed launches after
+    ``warmup`` untimed ones, preceded by the same load phase."""
     import torch
 
     from pylbm_amd import backends as B
-    from pylbm_amd.benchmark import time_steps
+    from pylbm_amd.benchmark import gpu_state, state_delta, time_leg, time_steps, warm_to_load
 
     s = B.make("hip", variant, problem, init="none")
     s.init_equilibrium(*init)
-    t = time_steps(s, steps, warmup, repeats)
-    ev = statistics.median(t["event_s"])
+    if steps is None:
+        t = time_leg(s, proto["min_time"], proto["repeats"], proto["warm_time"])
+    else:
+        warm = warm_to_load(s, proto["warm_time"])
+        s0 = gpu_state()
+        t = time_steps(s, steps, warmup, repeats=3)
+        t["warm_steps"] = warm
+        t["machine"] = state_delta(s0, gpu_state())
     n = problem.n_nodes
+    ml = [n * t["steps"] / e / 1e6 for e in t["event_s"]]
+    ev = statistics.median(t["event_s"])
     rec = {
         "variant": variant, "collision": problem.collision, "lattice": problem.lattice.name,
         "size": list(problem.shape), "storage": s.storage, "steps": t["steps"],
-        "MLUPS": n * t["steps"] / ev / 1e6,
+        "repeats": len(t["event_s"]), "warm_steps": t["warm_steps"],
+        "MLUPS": statistics.median(ml), "MLUPS_min": min(ml), "MLUPS_max": max(ml),
         "MLUPS_wall": n * t["steps"] / statistics.median(t["wall_s"]) / 1e6,
         "us_per_step": ev / t["steps"] * 1e6, "bytes_per_update": s.bytes_per_node,
+        "machine": t["machine"],
     }
     rec["GBs"] = rec["MLUPS"] * 1e6 * s.bytes_per_node / 1e9
     ok = np.isfinite(s.reduce("mass"))
@@ -121,7 +142,7 @@ def sphere_array_box(n: int, d: int = 32, pitch: int = 64):
                    force=(1e-6, 0.0, 0.0), solid=solid)
 
 
-def geometry_legs(n2: int, steps: int, warmup: int) -> list:
+def geometry_legs(n2: int, proto: dict) -> list:
     """Obstacles + inlet + outlet (k_step_geo): the cylinder-array channel at the headline
     size and a sphere-array box in 3D; bytes per update include the link mask."""
     import torch
@@ -129,7 +150,7 @@ def geometry_legs(n2: int, steps: int, warmup: int) -> list:
     out = []
     p2 = cylinder_array_channel(n2)
     for v in ("gather", "fused-fp16", "fused-bf16"):
-        r = run_variant(v, p2, (None, None), steps, warmup)
+        r = run_variant(v, p2, (None, None), proto)
         r["workload"] = f"cylinder-array channel {n2}^2 (inlet, outlet, " \
                         f"{100 * p2.solid.mean():.1f}% solid)"
         r["bytes_per_update_with_mask"] = r["bytes_per_update"] + 2
@@ -139,7 +160,7 @@ def geometry_legs(n2: int, steps: int, warmup: int) -> list:
     n3 = 256
     p3 = sphere_array_box(n3)
     for v in ("gather", "fused-fp16"):
-        r = run_variant(v, p3, (None, None), 100, 10, repeats=2)
+        r = run_variant(v, p3, (None, None), proto)
         r["workload"] = f"sphere-array box {n3}^3 ({100 * p3.solid.mean():.1f}% solid)"
         r["bytes_per_update_with_mask"] = r["bytes_per_update"] + 4
         out.append(r)
@@ -169,6 +190,7 @@ def adjoint_leg(name: str, repeats: int = 3) -> dict:
 
     from pylbm_amd import backends as B
     from pylbm_amd.adjoint import RolloutConfig, rollout, velocity_loss
+    from pylbm_amd.benchmark import warm_to_load
     from pylbm_amd.fields import divergence_free_field, smooth_random_field
     from pylbm_amd.problem import Problem
 
@@ -193,8 +215,7 @@ def adjoint_leg(name: str, repeats: int = 3) -> dict:
     # forward only: the same kernels, T steps, no checkpoints
     s = B.make("hip", variant, p, init="none")
     s.init_equilibrium(None, u0)
-    s.step(T // 10)
-    s.synchronize()
+    warm_to_load(s, 0.25)  # the forward-only rate is taken at load, as every other leg
     fwd = []
     for _ in range(repeats):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -245,6 +266,37 @@ def cpu_baseline(n: int, steps: int) -> dict:
             "seconds": dt}
 
 
+def north_star_ratios(variants: list, adjoint: list, head: dict, copy_gbs: float) -> dict:
+    """The north-star targets computed from this run's own legs: fp32 fraction of the copy
+    ceiling, 16-bit over fp32 MLUPS per collision and pattern (legs of the same protocol),
+    and forward+adjoint over forward-only."""
+    def leg(variant, coll, lattice="D2Q9"):
+        for r in variants:
+            if (r["variant"] == variant and r["collision"] == coll and r["lattice"] == lattice
+                    and r.get("protocol") != "headline" and "workload" not in r):
+                return r["MLUPS"]
+        return None
+
+    def ratio(a, b):
+        return None if a is None or b is None else a / b
+
+    out = {"d2q9_fp32_frac_of_copy": head["GBs"] / copy_gbs,
+           "fp16_over_fp32_target": 1.8, "fwd_adjoint_target": 0.25}
+    for st in ("fp16", "bf16"):
+        for coll in ("bgk", "trt", "mrt"):
+            out[f"d2q9_{st}_over_fp32_{coll}"] = ratio(leg(f"fused-{st}", coll),
+                                                       leg("fused", coll))
+            out[f"d2q9_aa_{st}_over_fp32_{coll}"] = ratio(leg(f"fused-aa-{st}", coll),
+                                                          leg("fused-aa", coll))
+        out[f"d3q19_{st}_over_fp32_trt"] = ratio(leg(f"fused-{st}", "trt", "D3Q19"),
+                                                 leg("fused", "trt", "D3Q19"))
+        out[f"d3q19_aa_{st}_over_fp32_trt"] = ratio(leg(f"fused-aa-{st}", "trt", "D3Q19"),
+                                                    leg("fused-aa", "trt", "D3Q19"))
+    for r in adjoint:
+        out[f"fwd_adjoint_ratio_{r['case']}"] = r["ratio"]
+    return out
+
+
 def load_traffic() -> dict | None:
     p = ROOT / "profiles" / "traffic_d2q9_4096_f32_bgk.json"
     if p.exists():
@@ -270,6 +322,12 @@ def main() -> int:
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only the forward+adjoint legs (prints them, no JSON line)")
     ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--min-time", type=float, default=0.1,
+                    help="every non-headline leg: seconds per timed window (default 0.1)")
+    ap.add_argument("--repeats", type=int, default=5,
+                    help="every non-headline leg: timed windows, median reported (default 5)")
+    ap.add_argument("--warm-time", type=float, default=0.25,
+                    help="seconds of load before any leg's timing (default 0.25)")
     ap.add_argument("--n2", type=int, default=4096)
     ap.add_argument("--n3", type=int, default=512)
     a = ap.parse_args()
@@ -281,7 +339,7 @@ def main() -> int:
     import torch
 
     from pylbm_amd import _lib
-    from pylbm_amd.benchmark import SPEC_BANDWIDTH_GBS, copy_bandwidth
+    from pylbm_amd.benchmark import SPEC_BANDWIDTH_GBS, copy_bandwidth, gpu_state
 
     if not torch.cuda.is_available():
         print("bench.py needs a HIP GPU", file=sys.stderr)
@@ -345,18 +403,20 @@ def main() -> int:
     log(f"copy at the headline's size ({same['mib']} MiB each way): "
         f"{same['hip_copy_GBs']:.0f} GB/s")
 
-    head = run_variant("fused", problem_2d("bgk", n, rho, u), (rho, u), a.steps, a.warmup)
-    log(f"headline hip/fused BGK {n}^2: {head['MLUPS']:.0f} MLUPS, {head['GBs']:.0f} GB/s")
+    proto = {"min_time": a.min_time, "repeats": a.repeats, "warm_time": a.warm_time}
+    head = run_variant("fused", problem_2d("bgk", n, rho, u), (rho, u), proto,
+                       steps=a.steps, warmup=a.warmup)
+    head["protocol"] = "headline"
+    log(f"headline hip/fused BGK {n}^2: {head['MLUPS']:.0f} MLUPS, {head['GBs']:.0f} GB/s "
+        f"({head['steps']} timed steps)")
     variants = [head]
     if not a.quick:
         for coll in ("bgk", "trt", "mrt"):
             for v in ("fused", "fused-f32s", "fused-aa", "fused-fp16", "fused-bf16",
                       "fused-aa-fp16", "fused-aa-bf16"):
-                if coll == "bgk" and v == "fused":
-                    continue
                 if coll != "bgk" and v == "fused-f32s":
                     continue
-                r = run_variant(v, problem_2d(coll, n, rho, u), (rho, u), a.steps, a.warmup)
+                r = run_variant(v, problem_2d(coll, n, rho, u), (rho, u), proto)
                 variants.append(r)
                 log(f"  {coll} hip/{v}: {r['MLUPS']:.0f} MLUPS {r['GBs']:.0f} GB/s")
         from pylbm_amd.problem import Problem
@@ -366,12 +426,12 @@ def main() -> int:
         init3 = tgv_init_device(n3)
         for v in ("fused", "fused-fp16", "fused-bf16", "fused-aa", "fused-aa-fp16",
                   "fused-aa-bf16"):
-            r = run_variant(v, p3, init3, 100, 10, repeats=2)
+            r = run_variant(v, p3, init3, proto)
             variants.append(r)
             log(f"  D3Q19 {n3}^3 trt hip/{v}: {r['MLUPS']:.0f} MLUPS {r['GBs']:.0f} GB/s")
         del init3
         torch.cuda.empty_cache()
-        variants += geometry_legs(n, a.steps, a.warmup)
+        variants += geometry_legs(n, proto)
     for r in variants:
         r["pct_copy"] = 100.0 * r["GBs"] / cbw
         r["pct_spec"] = 100.0 * r["GBs"] / SPEC_BANDWIDTH_GBS
@@ -383,6 +443,8 @@ def main() -> int:
             adjoint.append(r)
             log(f"  fwd+adjoint {case}: {r['fwd_adjoint_MLUPS']:.0f} MLUPS vs forward "
                 f"{r['forward_MLUPS']:.0f} ({100 * r['ratio']:.1f}%)")
+
+    targets = north_star_ratios(variants, adjoint, head, cbw)
 
     t_launch = head["us_per_step"] * 1e-6
     achieved = head["bytes_per_update"] * n * n / t_launch / 1e9
@@ -417,6 +479,13 @@ def main() -> int:
         "cpu_baseline": cpu,
         "copy_ceiling": ceil,
         "pct_copy": head["pct_copy"],
+        "targets": targets,
+        "protocol": {"headline": f"{a.warm_time} s of load, {a.warmup} untimed steps, then "
+                                 f"3 windows of exactly {head['steps']} steps (median)",
+                     "legs": f"{a.warm_time} s of load, then {a.repeats} windows of one "
+                             f"calibrated step count, each >= {a.min_time} s (median; "
+                             "pylbm/benchmark.py:159-180)"},
+        "machine": gpu_state(),
         "variants": variants,
         "fwd_adjoint": adjoint,
         "device": dev,

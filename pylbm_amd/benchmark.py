@@ -67,6 +67,111 @@ def copy_bandwidth(mib: int = 4096, repeats: int = 20) -> dict:
     return out
 
 
+_SMI: dict[str, Any] = {}
+
+
+def gpu_state() -> dict:
+    """Read-only machine snapshot of GPU 0 (pylbm/benchmark.py:121-130 reads the same
+    quantities from nvidia-smi): graphics and memory clocks (MHz), socket power (W), hotspot
+    and HBM temperatures (C), and the firmware's accumulated power- and thermal-limit
+    counters (their difference over a timed window says whether a limiter was active).
+
+    Uses the ``amdsmi`` Python module when it is importable; returns ``{}`` otherwise or on
+    any error, so a missing tool never stops a measurement.  The firmware refreshes these
+    values every few tens of milliseconds, so they describe a leg, not a launch.
+    """
+    try:
+        import amdsmi
+
+        if "h" not in _SMI:
+            amdsmi.amdsmi_init()
+            _SMI["h"] = amdsmi.amdsmi_get_processor_handles()[0]
+        h = _SMI["h"]
+        m = amdsmi.amdsmi_get_gpu_metrics_info(h)
+        out = {"gfxclk_MHz": m.get("current_gfxclk"), "uclk_MHz": m.get("current_uclk"),
+               "socket_power_W": m.get("current_socket_power"),
+               "hotspot_C": m.get("temperature_hotspot"), "hbm_C": m.get("temperature_mem")}
+        try:
+            v = amdsmi.amdsmi_get_violation_status(h)
+            for k in ("acc_counter", "acc_ppt_pwr", "acc_socket_thrm", "acc_hbm_thrm",
+                      "acc_vr_thrm", "acc_prochot_thrm"):
+                if isinstance(v.get(k), int):
+                    out[k] = v[k]
+        except Exception:  # noqa: BLE001 - older firmware has no violation table
+            pass
+        return {k: val for k, val in out.items() if isinstance(val, (int, float))}
+    except Exception:  # noqa: BLE001 - a snapshot must never fail a measurement
+        return {}
+
+
+def state_delta(s0: dict, s1: dict) -> dict:
+    """Snapshot after a window, plus the growth of each accumulated limiter counter."""
+    out = dict(s1)
+    for k in s1:
+        if k.startswith("acc_") and k in s0:
+            out[k] = s1[k] - s0[k]
+    return out
+
+
+def warm_to_load(solver: Any, seconds: float) -> int:
+    """Keep the GPU busy with the solver's own kernel for at least ``seconds`` of wall time.
+
+    On MI355X the 16-bit step kernels run up to 30% slower for about 30 ms after the GPU
+    leaves idle (a power-management transient of the firmware: the kernel's duration on one
+    fixed input rises from 102 to 133 us and recovers, while 150 ms of any prior load
+    removes it; fp32 kernels are not affected).  A leg therefore starts its timed windows
+    only after a load phase measured in time, not in steps.  Returns the steps taken.
+    """
+    m = solver.step_multiple
+    chunk, done = 2 * m, 0
+    t0 = time.perf_counter()
+    while True:
+        solver.step(chunk)
+        solver.synchronize()
+        done += chunk
+        dt = time.perf_counter() - t0
+        if dt >= seconds:
+            return done
+        want = (seconds - dt) * done / max(dt, 1e-6)  # steps that fill the rest at this rate
+        chunk = max(m, min(8 * chunk, math.ceil(want / m) * m))
+
+
+def time_leg(solver: Any, min_time: float = 0.1, repeats: int = 5,
+             warm_time: float = 0.25) -> dict:
+    """One benchmark leg: a load phase of ``warm_time`` seconds, then ``repeats`` timed
+    windows of the same step count, each at least ``min_time`` long (the protocol of
+    pylbm/benchmark.py:159-180 with HIP events on the launch stream).
+
+    Returns the per-window event and wall times, the steps per window and the machine
+    state (clocks, power, temperatures, limiter counters over the windows).
+    """
+    torch = _torch()
+    m = solver.step_multiple
+    warm_steps = warm_to_load(solver, warm_time)
+    # calibrate: the warm phase's rate sizes a window of >= min_time with 10% margin
+    probe = max(m, 2 * m)
+    t0 = time.perf_counter()
+    solver.step(probe)
+    solver.synchronize()
+    dt = max(time.perf_counter() - t0, 1e-6)
+    steps = max(probe, math.ceil(1.1 * min_time / dt * probe / m) * m)
+    s0 = gpu_state()
+    ev, wall = [], []
+    for _ in range(repeats):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e0.record()
+        solver.step(steps)
+        e1.record()
+        torch.cuda.synchronize()
+        wall.append(time.perf_counter() - t0)
+        ev.append(e0.elapsed_time(e1) / 1e3)
+    return {"steps": steps, "event_s": ev, "wall_s": wall, "warm_steps": warm_steps,
+            "machine": state_delta(s0, gpu_state())}
+
+
 def time_steps(solver: Any, steps: int, warmup: int, repeats: int = 1) -> dict:
     """Event-timed ``solver.step(steps)`` after ``warmup`` untimed steps.
 
@@ -136,8 +241,7 @@ def time_solver(solver: Any, min_time: float, repeats: int) -> tuple[list[float]
     tenth of ``min_time``; the rate it measures then sizes the repeat with 20% margin.
     """
     m = solver.step_multiple
-    solver.step(2 * m)  # warm-up
-    solver.synchronize()
+    warm_to_load(solver, 0.25)  # warm-up: leaves the GPU at load (see warm_to_load)
     probe = 2 * m
     while True:
         dt = _wall(solver, probe)
@@ -165,6 +269,7 @@ def measure(variant: str, dim: int, n: int, plan: dict, copy_bw: float,
         log(f"  skip hip/{variant} {n}^{dim}: {why}")
         return None
     s = cls(p)
+    state0 = gpu_state()
     times, steps = time_solver(s, plan["min_time"], plan["repeats"])
     ml = [p.n_nodes * steps / t / 1e6 for t in times]
     med = statistics.median(ml)
@@ -176,7 +281,10 @@ def measure(variant: str, dim: int, n: int, plan: dict, copy_bw: float,
            "bytes_per_unit": s.bytes_per_node, "effective_GBs": eff,
            "pct_copy_bw": 100.0 * eff / copy_bw, "pct_spec_bw": 100.0 * eff / SPEC_BANDWIDTH_GBS,
            "flops_per_unit": None, "pct_peak_flops": None,
-           "memory_MiB": s.memory_bytes() / 2**20, "notes": cls.description}
+           "memory_MiB": s.memory_bytes() / 2**20,
+           "notes": "; ".join([cls.description,
+                               f"gfx clock {state0.get('gfxclk_MHz', '?')} MHz, "
+                               f"hotspot {state0.get('hotspot_C', '?')} C"])}
     del s
     torch.cuda.empty_cache()
     log(f"  hip {variant:14s} {collision} {'x'.join(map(str, p.shape)):>14s} {med:10.1f} MLUPS "

@@ -1,6 +1,7 @@
 // C ABI of libpylbm_hip.so (declared in include/pylbm_hip.h): parameter checks, kernel
 // argument setup, the C-side step loop and error reporting.  Device memory is owned by the
-// caller (in pylbm's hip backend: torch tensors); nothing here allocates.
+// caller (in pylbm's hip backend: torch tensors); the only allocation here is the
+// stream-ordered per-workgroup partials buffer of pylbm_hip_reduce.
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -86,6 +87,10 @@ int check_params(const pylbm_params* p, const char* what) {
   const int last = p->lattice == PYLBM_D2Q9 ? 1 : 2;
   if (p->ulid != 0.0f && p->periodic[last])
     return fail(PYLBM_EINVAL, "%s: a moving lid needs a wall on the last axis", what);
+  // fp32 side arrays (the per-cell force field) use 32-bit byte offsets as well
+  if (p->force_field && N * 4 >= ((int64_t)1 << 32))
+    return fail(PYLBM_ERANGE, "%s: a %lld-node fp32 force-field plane exceeds the 4 GiB "
+                "byte offsets of the kernels", what, (long long)N);
   if (p->lattice == PYLBM_D2Q9 && p->force_field == nullptr && p->force[2] != 0.0f)
     return fail(PYLBM_EINVAL, "%s: D2Q9 has no z force", what);
   if (p->lam_storage != PYLBM_F32 && p->lam_storage != PYLBM_F16)
@@ -333,7 +338,17 @@ int pylbm_hip_reduce(const pylbm_params* p, int kind, const void* f, const void*
   x.d_in = kind == PYLBM_RED_LOSS ? static_cast<const double*>(other) : nullptr;
   x.kind = kind;
   x.d_out = out;
-  return hip_fail(launch_aux(kOpReduce, x, (hipStream_t)stream), "pylbm_hip_reduce");
+  // the per-workgroup partials live for the two launches only: a stream-ordered
+  // allocation, so concurrent reductions on other streams never share them
+  hipStream_t s = (hipStream_t)stream;
+  void* partials = nullptr;
+  if (hipError_t e = hipMallocAsync(&partials, sizeof(double) * reduce_blocks(p->n_nodes), s))
+    return hip_fail((int)e, "pylbm_hip_reduce (partials)");
+  x.partials = static_cast<double*>(partials);
+  const int e = launch_aux(kOpReduce, x, s);
+  const hipError_t ef = hipFreeAsync(partials, s);
+  if (e) return hip_fail(e, "pylbm_hip_reduce");
+  return hip_fail((int)ef, "pylbm_hip_reduce (partials)");
 }
 
 int pylbm_hip_links(const pylbm_params* p, const uint8_t* solid, void* links, void* stream) {
@@ -363,6 +378,10 @@ int64_t pylbm_hip_adjoint_workspace(const pylbm_params* p) {
 
 static int check_adjoint(const pylbm_params* p, const char* what) {
   if (int e = check_params(p, what)) return e;
+  // lam (fp32 by default), dF and grad_u are fp32 planes indexed with 32-bit byte offsets
+  if (p->n_nodes * 4 >= ((int64_t)1 << 32))
+    return fail(PYLBM_ERANGE, "%s: a %lld-node fp32 adjoint plane exceeds the 4 GiB byte "
+                "offsets of the kernels", what, (long long)p->n_nodes);
   if (p->aa) return fail(PYLBM_EINVAL, "%s: the adjoint runs on the two-buffer layout", what);
   if (p->collision == PYLBM_MRT)
     return fail(PYLBM_ENOTSUP, "%s: the adjoint covers BGK and TRT", what);

@@ -162,37 +162,38 @@ __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   return v;
 }
+// max that keeps a NaN from either side (fmax would drop it), so a NaN or an Inf - Inf
+// difference makes MAXABS_DIFF NaN instead of hiding it
+__device__ __forceinline__ double max_nan(double a, double b) {
+  return (a > b || a != a) ? a : b;
+}
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  for (int off = 32; off > 0; off >>= 1) v = max_nan(v, __shfl_down(v, off, 64));
   return v;
 }
 
+// Sum (or max) over the workgroup in a fixed order; thread 0 returns the result.
 template <bool MAX>
-__device__ void block_reduce_to(double v, double* out) {
+__device__ double block_reduce(double v) {
   __shared__ double part[kBlock / 64];
   v = MAX ? wave_max(v) : wave_sum(v);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (lane == 0) part[wv] = v;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double t = part[0];
-    for (int k = 1; k < kBlock / 64; ++k) t = MAX ? fmax(t, part[k]) : t + part[k];
-    if (MAX) {
-      // non-negative doubles order like their bit patterns
-      atomicMax((unsigned long long*)out, (unsigned long long)__double_as_longlong(t));
-    } else {
-      atomicAdd(out, t);
-    }
-  }
+  double t = part[0];
+  if (threadIdx.x == 0)
+    for (int k = 1; k < kBlock / 64; ++k) t = MAX ? max_nan(t, part[k]) : t + part[k];
+  return t;
 }
 
 template <class L, int ST>
 __global__ void __launch_bounds__(kBlock)
 k_reduce(const typename Codec<ST>::T* __restrict__ A, const typename Codec<ST>::T* __restrict__ B,
-         const double* __restrict__ target, MomArgs m, int kind, double* out) {
-  // grid-stride over the nodes with a bounded grid, so that one float64 atomic per
-  // workgroup (a few thousand in all) is all that serialises at the target address
+         const double* __restrict__ target, MomArgs m, int kind, double* __restrict__ partials) {
+  // grid-stride over the nodes with a grid that depends on N only; each workgroup writes
+  // its partial to its own slot and k_reduce_final combines the slots in a fixed order,
+  // so the float64 result is the same on every run (no atomics)
   double v = 0.0;
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   for (int64_t nn = (int64_t)blockIdx.x * kBlock + threadIdx.x; nn < m.N; nn += stride) {
@@ -232,16 +233,22 @@ k_reduce(const typename Codec<ST>::T* __restrict__ A, const typename Codec<ST>::
       for (int i = 0; i < L::Q; ++i) {
         const double a = (double)Codec<ST>::ld(A[(size_t)i * m.N + n]);
         const double b = (double)Codec<ST>::ld(B[(size_t)i * m.N + n]);
-        v = fmax(v, fabs(a - b));
+        v = max_nan(v, fabs(a - b));
       }
     }
   }
-  if (kind == PYLBM_RED_MAXABS_DIFF) block_reduce_to<true>(v, out);
-  else block_reduce_to<false>(v, out);
+  const double t = kind == PYLBM_RED_MAXABS_DIFF ? block_reduce<true>(v) : block_reduce<false>(v);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
-__global__ void k_zero1(double* out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *out = 0.0;
+// One workgroup: combines the n partials of k_reduce in a fixed order into *out.
+template <bool MAX>
+__global__ void __launch_bounds__(kBlock)
+k_reduce_final(const double* __restrict__ partials, int n, double* __restrict__ out) {
+  double v = 0.0;
+  for (int k = threadIdx.x; k < n; k += kBlock) v = MAX ? max_nan(v, partials[k]) : v + partials[k];
+  const double t = block_reduce<MAX>(v);
+  if (threadIdx.x == 0) *out = t;
 }
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -292,11 +299,17 @@ int aux_typed(int op, const AuxArgs& x, hipStream_t s) {
     case kOpMacro:
       k_macro<L, ST><<<g, b, 0, s>>>((const T*)x.f, m, x.d_out, x.d_out2);
       break;
-    case kOpReduce:
-      k_zero1<<<1, 64, 0, s>>>(x.d_out);
-      k_reduce<L, ST><<<dim3(g.x < 4096u ? g.x : 4096u), b, 0, s>>>(
-          (const T*)x.f, (const T*)x.f2, x.d_in, m, x.kind, x.d_out);
+    case kOpReduce: {
+      const int nb = reduce_blocks(N);
+      if (!x.partials) return PYLBM_EINVAL;
+      k_reduce<L, ST><<<dim3(nb), b, 0, s>>>((const T*)x.f, (const T*)x.f2, x.d_in, m, x.kind,
+                                              x.partials);
+      if (x.kind == PYLBM_RED_MAXABS_DIFF)
+        k_reduce_final<true><<<1, b, 0, s>>>(x.partials, nb, x.d_out);
+      else
+        k_reduce_final<false><<<1, b, 0, s>>>(x.partials, nb, x.d_out);
       break;
+    }
     default:
       return PYLBM_EINVAL;
   }
@@ -318,6 +331,11 @@ int aux_lat(int op, const AuxArgs& x, hipStream_t s) {
 
 int launch_aux(int op, const AuxArgs& x, hipStream_t s) {
   return x.lattice == PYLBM_D2Q9 ? aux_lat<D2Q9>(op, x, s) : aux_lat<D3Q19>(op, x, s);
+}
+
+int reduce_blocks(int64_t n_nodes) {
+  const int64_t nb = (n_nodes + kBlock - 1) / kBlock;
+  return (int)(nb < kReducePartials ? nb : kReducePartials);
 }
 
 int launch_copy(const void* src, void* dst, int64_t nbytes, hipStream_t s) {

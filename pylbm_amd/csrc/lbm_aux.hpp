@@ -21,9 +21,15 @@ struct AuxArgs {
   const double* d_in2;      // u
   double* d_out;            // canonical state / rho / reduction result
   double* d_out2;           // u
+  double* partials;         // reduction: one double per workgroup (reduce_blocks)
 };
 
+// A reduction runs on min(ceil(N / kBlock), kReducePartials) workgroups, one float64
+// partial each.
+constexpr int kReducePartials = 4096;
+
 int launch_aux(int op, const AuxArgs& x, hipStream_t s);
+int reduce_blocks(int64_t n_nodes);
 int launch_copy(const void* src, void* dst, int64_t nbytes, hipStream_t s);
 
 }  // namespace plbm

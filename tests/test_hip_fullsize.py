@@ -114,7 +114,7 @@ def _tgv512():
     return u
 
 
-@pytest.mark.parametrize("storage", ["", "-fp16"])
+@pytest.mark.parametrize("storage", ["", "-fp16", "-bf16"])
 def test_d3q19_512_aa_matches_two_buffer_and_conserves_mass(storage):
     import torch
 

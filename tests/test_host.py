@@ -239,3 +239,31 @@ def test_abi_rejects_bad_calls():
     assert "lid" in lib.pylbm_hip_last_error().decode()
     big = _params(nx=65536, ny=65536, n_nodes=65536 * 65536)
     assert lib.pylbm_hip_step(ctypes.byref(big), a, b, _lib.AB, None) == 1002
+
+
+def test_abi_rejects_fp32_side_planes_beyond_4gib():
+    """A 16-bit state of more than 2^30 nodes fits the kernels' 32-bit byte offsets, but
+    its fp32 force field and fp32 adjoint planes (4 bytes per node) do not: such calls are
+    rejected before any HIP call instead of wrapping the offsets."""
+    lib = _lib.load()
+    a, b = ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000)
+    nx, ny = 65536, 16385  # N = 1,073,807,360 > 2^30, N * 2 < 2^32
+    n = nx * ny
+    ok16 = _params(storage=_lib.F16, nx=nx, ny=ny, n_nodes=n)
+    # the 16-bit state alone passes the checks (the workspace query makes no HIP call)
+    assert lib.pylbm_hip_adjoint_workspace(ctypes.byref(ok16)) > 0
+    field = _params(storage=_lib.F16, nx=nx, ny=ny, n_nodes=n, force_field=0x5000)
+    assert lib.pylbm_hip_step(ctypes.byref(field), a, b, _lib.AB, None) == 1002
+    assert "force-field" in lib.pylbm_hip_last_error().decode()
+    assert lib.pylbm_hip_run(ctypes.byref(field), a, b, 2, None) == 1002
+    for lam in (_lib.F32, _lib.F16):
+        adj = _params(storage=_lib.F16, nx=nx, ny=ny, n_nodes=n, lam_storage=lam)
+        assert lib.pylbm_hip_adjoint_step(ctypes.byref(adj), a, b, ctypes.c_void_p(0x3000),
+                                          None, ctypes.c_void_p(0x4000), None) == 1002
+        assert "adjoint plane" in lib.pylbm_hip_last_error().decode()
+        assert lib.pylbm_hip_adjoint_seed(ctypes.byref(adj), a, b, ctypes.c_void_p(0x3000),
+                                          None, None) == 1002
+    # just under the limit: 2^30 - 256 nodes with a force field pass the range checks
+    under = _params(storage=_lib.F16, nx=65536, ny=16383, n_nodes=65536 * 16383,
+                    force_field=0x5000)
+    assert lib.pylbm_hip_adjoint_workspace(ctypes.byref(under)) > 0

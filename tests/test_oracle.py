@@ -313,3 +313,30 @@ def test_encode_decode_roundtrip(storage, aa):
     rest = R.equilibrium(lat, np.ones((3, 4)), np.zeros((2, 3, 4)))
     if storage != "f32":
         assert not np.any(R.encode(rest, storage, aa, lat).astype(np.float64))
+
+
+@pytest.mark.parametrize("dim,periodic,u_lid,force", [
+    (2, (True, True), 0.0, None), (2, (False, False), 0.05, None),
+    (2, (True, False), 0.0, (1e-5, 0.0)), (3, (False, True, False), 0.05, None)])
+def test_aa_launches_ignore_a_ghost_offset_at_tau_one(dim, periodic, u_lid, force):
+    """Premise of the GPU write-every-slot test of the AA kernels: with BGK at tau = 1 an AA
+    launch maps X and X + eps g (g the constant ghost field of tests/test_hip_poison.py) to
+    the same array, while leaving a slot unwritten would keep its offset eps g_i."""
+    from tests.test_hip_poison import _ghost
+    from pylbm_amd.lattice import LATTICES
+
+    shape = (9, 7) if dim == 2 else (6, 5, 4)
+    case = R.Case(shape=shape, tau=1.0, periodic=periodic, u_lid=u_lid, force=force)
+    lat = case.lat
+    g = _ghost(LATTICES["D2Q9" if dim == 2 else "D3Q19"])
+    rng = np.random.default_rng(1)
+    X = R.equilibrium(lat, 1 + 0.01 * rng.standard_normal(case.array_shape),
+                      [0.03 * rng.standard_normal(case.array_shape) for _ in range(dim)])
+    X = X + 0.005 * rng.standard_normal(X.shape)
+    Xg = X + 0.025 * g.reshape((-1,) + (1,) * dim)
+    for launch in (R.aa_odd, R.aa_even):
+        a, b = launch(X, case), launch(Xg, case)
+        assert np.abs(a - b).max() < 1e-14
+        # a launch that left a slot unwritten would return the two inputs there, which
+        # differ by eps |g_i| >= 0.025 in every slot
+        assert np.abs(Xg - X).min() >= 0.025 - 1e-12

@@ -163,6 +163,19 @@ int pylbm_hip_reduce(const pylbm_params* p, int kind, const void* f, const void*
                      double* out, void* stream);
 
 /*
+ * pylbm_hip_reduce without any allocation: the same kernels and the same result, bit for
+ * bit, with the per-workgroup partials kept in a static device array of the library
+ * (4096 doubles, 32 KiB per GPU, held until the process ends) instead of a stream-ordered
+ * allocation.  It calls no hipMalloc, hipMallocAsync or hipFree and has no counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops).  Every problem size the library accepts fits (the reduction grid
+ * is capped at 4096 workgroups).  All calls on one GPU share its array: calls that overlap
+ * on one GPU (different streams or host threads) race; serialise them.
+ */
+int pylbm_hip_reduce_noalloc(const pylbm_params* p, int kind, const void* f,
+                             const void* other, double* out, void* stream);
+
+/*
  * Per-node solid-source mask of a solid field: links[n] bit i (i >= 1) is set when the
  * node x - c_i, wrapped periodically on every axis, is solid (solid: N bytes, nonzero =
  * solid; links: N uint16 for D2Q9, N uint32 for D3Q19).  Pass links in params.links to

@@ -1,7 +1,8 @@
 // C ABI of libpylbm_hip.so (declared in include/pylbm_hip.h): parameter checks, kernel
 // argument setup, the C-side step loop and error reporting.  Device memory is owned by the
 // caller (in pylbm's hip backend: torch tensors); the only allocation here is the
-// stream-ordered per-workgroup partials buffer of pylbm_hip_reduce.
+// stream-ordered per-workgroup partials buffer of pylbm_hip_reduce (its allocation-free
+// variant, pylbm_hip_reduce_noalloc, uses a static device array instead).
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -322,22 +323,34 @@ int pylbm_hip_macroscopic(const pylbm_params* p, const void* f, double* rho, dou
   return hip_fail(launch_aux(kOpMacro, x, (hipStream_t)stream), "pylbm_hip_macroscopic");
 }
 
-int pylbm_hip_reduce(const pylbm_params* p, int kind, const void* f, const void* other,
-                     double* out, void* stream) {
-  if (int e = check_params(p, "pylbm_hip_reduce")) return e;
-  if (!f || !out) return fail(PYLBM_EINVAL, "pylbm_hip_reduce: NULL buffer");
+static int check_reduce(const pylbm_params* p, int kind, const void* f, const void* other,
+                        const double* out, const char* what) {
+  if (int e = check_params(p, what)) return e;
+  if (!f || !out) return fail(PYLBM_EINVAL, "%s: NULL buffer", what);
   if (kind < PYLBM_RED_MASS || kind > PYLBM_RED_DOT)
-    return fail(PYLBM_EINVAL, "pylbm_hip_reduce: unknown kind %d", kind);
+    return fail(PYLBM_EINVAL, "%s: unknown kind %d", what, kind);
   if ((kind == PYLBM_RED_MAXABS_DIFF || kind == PYLBM_RED_DOT) && !other)
-    return fail(PYLBM_EINVAL, "pylbm_hip_reduce: this kind needs a second state");
+    return fail(PYLBM_EINVAL, "%s: this kind needs a second state", what);
   if (kind == PYLBM_RED_LOSS && !other)
-    return fail(PYLBM_EINVAL, "pylbm_hip_reduce: LOSS needs the target velocity");
+    return fail(PYLBM_EINVAL, "%s: LOSS needs the target velocity", what);
+  return 0;
+}
+
+static AuxArgs reduce_args(const pylbm_params* p, int kind, const void* f, const void* other,
+                           double* out) {
   AuxArgs x = aux_from(p);
   x.f = f;
   x.f2 = kind == PYLBM_RED_LOSS ? nullptr : other;
   x.d_in = kind == PYLBM_RED_LOSS ? static_cast<const double*>(other) : nullptr;
   x.kind = kind;
   x.d_out = out;
+  return x;
+}
+
+int pylbm_hip_reduce(const pylbm_params* p, int kind, const void* f, const void* other,
+                     double* out, void* stream) {
+  if (int e = check_reduce(p, kind, f, other, out, "pylbm_hip_reduce")) return e;
+  AuxArgs x = reduce_args(p, kind, f, other, out);
   // the per-workgroup partials live for the two launches only: a stream-ordered
   // allocation, so concurrent reductions on other streams never share them
   hipStream_t s = (hipStream_t)stream;
@@ -349,6 +362,23 @@ int pylbm_hip_reduce(const pylbm_params* p, int kind, const void* f, const void*
   const hipError_t ef = hipFreeAsync(partials, s);
   if (e) return hip_fail(e, "pylbm_hip_reduce");
   return hip_fail((int)ef, "pylbm_hip_reduce (partials)");
+}
+
+int pylbm_hip_reduce_noalloc(const pylbm_params* p, int kind, const void* f,
+                             const void* other, double* out, void* stream) {
+  if (int e = check_reduce(p, kind, f, other, out, "pylbm_hip_reduce_noalloc")) return e;
+  // the grid is capped at kReducePartials workgroups, so every size check_params accepts
+  // fits the static array; the check guards that invariant
+  const int nb = reduce_blocks(p->n_nodes);
+  if (nb > kReducePartials)
+    return fail(PYLBM_ERANGE, "pylbm_hip_reduce_noalloc: %d partials exceed the %d of the "
+                "static scratch", nb, kReducePartials);
+  double* scratch_ptr = nullptr;
+  if (int e = reduce_scratch(&scratch_ptr))
+    return hip_fail(e, "pylbm_hip_reduce_noalloc (scratch address)");
+  AuxArgs x = reduce_args(p, kind, f, other, out);
+  x.partials = scratch_ptr;
+  return hip_fail(launch_aux(kOpReduce, x, (hipStream_t)stream), "pylbm_hip_reduce_noalloc");
 }
 
 int pylbm_hip_links(const pylbm_params* p, const uint8_t* solid, void* links, void* stream) {

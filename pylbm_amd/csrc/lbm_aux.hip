@@ -11,6 +11,8 @@
 //   moments : pylbm/core.py:151-169, 255-266
 #include "lbm_aux.hpp"
 
+#include <mutex>
+
 #pragma clang fp contract(off)
 
 namespace plbm {
@@ -336,6 +338,32 @@ int launch_aux(int op, const AuxArgs& x, hipStream_t s) {
 int reduce_blocks(int64_t n_nodes) {
   const int64_t nb = (n_nodes + kBlock - 1) / kBlock;
   return (int)(nb < kReducePartials ? nb : kReducePartials);
+}
+
+// The allocation-free reductions keep their partials in this array, one copy per GPU for
+// the life of the process.  Host code uses the address that hipGetSymbolAddress returns on
+// each device (reduce_scratch), never the array's name.
+static __device__ __align__(256) double scratch[kReducePartials];
+
+namespace {
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+int reduce_scratch(double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return (int)e;
+  if (dev < 0 || dev >= kMaxDevices) return PYLBM_ERANGE;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return (int)e;
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
 }
 
 int launch_copy(const void* src, void* dst, int64_t nbytes, hipStream_t s) {

@@ -25,11 +25,16 @@ struct AuxArgs {
 };
 
 // A reduction runs on min(ceil(N / kBlock), kReducePartials) workgroups, one float64
-// partial each.
+// partial each; the allocation-free entry keeps them in a static device array of this
+// size (32 KiB per GPU).
 constexpr int kReducePartials = 4096;
+constexpr int kMaxDevices = 64;
 
 int launch_aux(int op, const AuxArgs& x, hipStream_t s);
 int reduce_blocks(int64_t n_nodes);
+// Device address of the current GPU's static partials array, looked up with
+// hipGetSymbolAddress on the first call on each device; 0 or an error code.
+int reduce_scratch(double** out);
 int launch_copy(const void* src, void* dst, int64_t nbytes, hipStream_t s);
 
 }  // namespace plbm

@@ -18,7 +18,8 @@ step count, each at least ``--min-time`` seconds, median reported, with the GPU'
 power, temperatures and limiter counters recorded per leg.
 
 ``--profile [d2q9|d3q19|d3q19-fp16|d3q19-aa|d3q19-aa-fp16|geometry-d2q9|geometry-d3q19]`` runs
-only one step kernel (the headline by default) and the copy kernel (for rocprofv3).
+only one step kernel (the headline by default) and the copy kernel (for rocprofv3);
+``--profile copy`` runs every probe of the copy ceiling over 4 GiB buffers.
 """
 
 from __future__ import annotations
@@ -312,7 +313,7 @@ def main() -> int:
     ap.add_argument("--quick", action="store_true", help="headline + copy ceiling only")
     ap.add_argument("--profile", nargs="?", const="d2q9", default=None,
                     choices=["d2q9", "d3q19", "d3q19-fp16", "d3q19-aa", "d3q19-aa-fp16",
-                             "geometry-d2q9", "geometry-d3q19"],
+                             "geometry-d2q9", "geometry-d3q19", "copy"],
                     help="only one step kernel and the copy kernel, for rocprofv3: the headline "
                          "(d2q9, default), D3Q19 512^3 TRT fp32 / fp16, two-buffer or AA, or "
                          "the geometry kernel (fp16) on the cylinder-array channel 4096^2 / "
@@ -354,6 +355,22 @@ def main() -> int:
                 f"{r['forward_MLUPS']:.0f} ({100 * r['ratio']:.1f}%)")
         return 0
     n = a.n2
+    if a.profile == "copy":
+        # the ceiling's probe family on 4 GiB buffers, a.steps launches of each
+        nbytes = 4 << 30
+        x = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        y = torch.empty_like(x)
+        st = int(torch.cuda.current_stream().cuda_stream)
+        for _ in range(a.steps):
+            _lib.call("pylbm_hip_copy", x.data_ptr(), y.data_ptr(), nbytes, st)
+        for kind in range(4):
+            for unroll in (1, 2, 4, 8):
+                for _ in range(a.steps):
+                    _lib.call("pylbm_hip_stream_probe", kind, unroll, x.data_ptr(),
+                              y.data_ptr(), nbytes, st)
+        torch.cuda.synchronize()
+        log(f"profile: {a.steps} launches of each stream probe over {nbytes} B done")
+        return 0
     rho, u = shear_wave_init(n, SEED)
 
     if a.profile:
@@ -395,13 +412,14 @@ def main() -> int:
     log("copy ceiling (4 GiB buffers)")
     ceil = copy_bandwidth(4096 if not a.quick else 1024)
     cbw = ceil["copy_bandwidth_GBs"]
-    log(f"copy ceiling {cbw:.0f} GB/s (hip copy {ceil['hip_copy_GBs']:.0f}, "
-        f"DtoD {ceil['memcpy_dtod_GBs']:.0f})")
+    log(f"copy ceiling {cbw:.0f} GB/s (best: {ceil['copy_best']}; library copy "
+        f"{ceil['hip_copy_GBs']:.0f}, DtoD {ceil['memcpy_dtod_GBs']:.0f}; read-only "
+        f"{ceil['read_only_GBs']:.0f}, write-only {ceil['write_only_GBs']:.0f})")
     # the same copy at the headline's own size (9 fp32 planes of n^2 each way), reported
     # beside the 4 GiB ceiling: the copy's bandwidth depends on its size
     same = copy_bandwidth(max(1, 9 * a.n2 * a.n2 * 4 // 2**20), repeats=20)
     log(f"copy at the headline's size ({same['mib']} MiB each way): "
-        f"{same['hip_copy_GBs']:.0f} GB/s")
+        f"{same['copy_bandwidth_GBs']:.0f} GB/s ({same['copy_best']})")
 
     proto = {"min_time": a.min_time, "repeats": a.repeats, "warm_time": a.warm_time}
     head = run_variant("fused", problem_2d("bgk", n, rho, u), (rho, u), proto,
@@ -457,8 +475,8 @@ def main() -> int:
             "algorithmic_bytes_per_launch": head["bytes_per_update"] * n * n,
             "avg_launch_us": t_launch * 1e6,
             "frac_of_measured_copy": achieved / cbw,
-            "copy_same_size_GBs": same["hip_copy_GBs"],
-            "frac_of_copy_same_size": achieved / same["hip_copy_GBs"]}
+            "copy_same_size_GBs": same["copy_bandwidth_GBs"],
+            "frac_of_copy_same_size": achieved / same["copy_bandwidth_GBs"]}
 
     cpu = None
     if not a.no_cpu and not a.quick:

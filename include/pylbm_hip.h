@@ -191,6 +191,17 @@ int pylbm_hip_links(const pylbm_params* p, const uint8_t* solid, void* links, vo
  */
 int pylbm_hip_copy(const void* src, void* dst, int64_t nbytes, void* stream);
 
+/*
+ * Probes of the bandwidth ceiling, nbytes a multiple of 16: kind 0 copy with non-temporal
+ * loads and stores (pylbm_hip_copy is kind 0 at unroll 1), 1 copy with ordinary loads and
+ * stores, 2 read-only stream of src (dst receives at most one 16-byte word per
+ * workgroup, only for contents a fill of 1.0f never has: give it nbytes / (256 unroll) + 16
+ * bytes), 3 write-only stream into dst (src unused); unroll = 1, 2, 4 or 8 16-byte accesses
+ * per lane.  Extends the copy probe of bench/roofline.py:30-51, which has one form only.
+ */
+int pylbm_hip_stream_probe(int kind, int unroll, const void* src, void* dst, int64_t nbytes,
+                           void* stream);
+
 /* ---------------------------------------------------------------------------------------
  * Discrete adjoint of the two-buffer step (BGK / TRT, Guo forcing; params->aa must be 0).
  * No reference counterpart: the reference has no gradient path (its torch rungs run under

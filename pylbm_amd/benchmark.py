@@ -24,46 +24,71 @@ def _torch():
     return torch
 
 
-def copy_bandwidth(mib: int = 4096, repeats: int = 20) -> dict:
+#: pylbm_hip_stream_probe kinds
+PROBE_KINDS = {"copy_nt": 0, "copy": 1, "read": 2, "write": 3}
+
+
+def _event_times(fn, repeats: int) -> list[float]:
+    torch = _torch()
+    for _ in range(3):
+        fn()
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(repeats):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        e1.synchronize()
+        times.append(e0.elapsed_time(e1) / 1e3)
+    return times
+
+
+def copy_bandwidth(mib: int = 4096, repeats: int = 20, sweep: bool = True) -> dict:
     """Median device copy bandwidth (read + write counted, bench/roofline.py:30-51).
 
-    Two probes over ``mib`` MiB buffers (default 4 GiB, far larger than the 256 MB
-    Infinity Cache): the library's streaming copy kernel and torch ``copy_``
-    (hipMemcpyDtoD).  Returns GB/s for both and the per-launch kernel time.
+    Probes over ``mib`` MiB buffers (default 4 GiB, far larger than the 256 MB Infinity
+    Cache): the library's streaming copy (``pylbm_hip_copy``), torch ``copy_``
+    (hipMemcpyDtoD) and, with ``sweep``, the probe family of ``pylbm_hip_stream_probe``:
+    copy with non-temporal or ordinary accesses at 1, 2, 4 and 8 16-byte accesses per lane,
+    and read-only / write-only streams (their bytes counted once).  The ceiling is the best
+    copy of them all; the read-only and write-only rates are reported beside it.
     """
     torch = _torch()
     nbytes = mib * 2**20
     a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
     b = torch.empty_like(a)
     stream = int(torch.cuda.current_stream().cuda_stream)
-
-    def hip_copy() -> None:
-        _lib.call("pylbm_hip_copy", a.data_ptr(), b.data_ptr(), nbytes, stream)
-
-    def dtod() -> None:
-        b.copy_(a)
-
+    probes: dict[str, tuple[Any, int]] = {
+        "hip_copy": (lambda: _lib.call("pylbm_hip_copy", a.data_ptr(), b.data_ptr(), nbytes,
+                                       stream), 2),
+        "memcpy_dtod": (lambda: b.copy_(a), 2),
+    }
+    if sweep:
+        for name, kind in PROBE_KINDS.items():
+            for u in (1, 2, 4, 8):
+                probes[f"{name}_u{u}"] = (
+                    (lambda k=kind, u=u: _lib.call("pylbm_hip_stream_probe", k, u,
+                                                   a.data_ptr(), b.data_ptr(), nbytes,
+                                                   stream)),
+                    2 if name.startswith("copy") else 1)
     out: dict[str, Any] = {"mib": mib}
-    for name, fn in (("hip_copy", hip_copy), ("memcpy_dtod", dtod)):
-        for _ in range(3):
-            fn()
-        torch.cuda.synchronize()
-        times = []
-        for _ in range(repeats):
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record()
-            fn()
-            e1.record()
-            e1.synchronize()
-            times.append(e0.elapsed_time(e1) / 1e3)
-        bw = [2 * nbytes / t / 1e9 for t in times]
+    for name, (fn, ways) in probes.items():
+        times = _event_times(fn, repeats)
+        bw = [ways * nbytes / t / 1e9 for t in times]
         out[f"{name}_GBs"] = statistics.median(bw)
         out[f"{name}_max_GBs"] = max(bw)
         out[f"{name}_ms"] = statistics.median(times) * 1e3
     del a, b
     torch.cuda.empty_cache()
-    out["copy_bandwidth_GBs"] = max(out["hip_copy_GBs"], out["memcpy_dtod_GBs"])
+    copies = [k for k in probes if probes[k][1] == 2]
+    best = max(copies, key=lambda k: out[f"{k}_GBs"])
+    out["copy_bandwidth_GBs"] = out[f"{best}_GBs"]
+    out["copy_best"] = best
+    if sweep:
+        out["read_only_GBs"] = max(out[f"read_u{u}_GBs"] for u in (1, 2, 4, 8))
+        out["write_only_GBs"] = max(out[f"write_u{u}_GBs"] for u in (1, 2, 4, 8))
     return out
 
 

@@ -399,6 +399,22 @@ int pylbm_hip_copy(const void* src, void* dst, int64_t nbytes, void* stream) {
   return hip_fail(launch_copy(src, dst, nbytes, (hipStream_t)stream), "pylbm_hip_copy");
 }
 
+int pylbm_hip_stream_probe(int kind, int unroll, const void* src, void* dst, int64_t nbytes,
+                           void* stream) {
+  if (kind < 0 || kind > 3) return fail(PYLBM_EINVAL, "pylbm_hip_stream_probe: kind 0..3");
+  if (unroll != 1 && unroll != 2 && unroll != 4 && unroll != 8)
+    return fail(PYLBM_EINVAL, "pylbm_hip_stream_probe: unroll must be 1, 2, 4 or 8");
+  if ((kind != 3 && !src) || !dst) return fail(PYLBM_EINVAL, "pylbm_hip_stream_probe: NULL buffer");
+  if (nbytes < 0 || nbytes % 16)
+    return fail(PYLBM_EINVAL, "pylbm_hip_stream_probe: nbytes must be a non-negative multiple "
+                "of 16");
+  if (kind == 2 && nbytes / 16 > (int64_t)kBlock * unroll * ((int64_t)1 << 31))
+    return fail(PYLBM_ERANGE, "pylbm_hip_stream_probe: too many workgroups");
+  if (nbytes == 0) return 0;
+  return hip_fail(launch_probe(kind, unroll, src, dst, nbytes, (hipStream_t)stream),
+                  "pylbm_hip_stream_probe");
+}
+
 int64_t pylbm_hip_adjoint_workspace(const pylbm_params* p) {
   if (check_params(p, "pylbm_hip_adjoint_workspace")) return -1;
   KArgs a;

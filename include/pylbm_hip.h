@@ -225,7 +225,9 @@ int pylbm_hip_adjoint_seed(const pylbm_params* p, const void* f_T, const float* 
 /* One adjoint step: lam_out = (d f^{t+1} / d f^t)^T lam_in, with f_t the stored state the
  * forward step read.  Accumulates dF += dL/dF(x) (force field only) and adds the step's
  * dL/ds+ and dL/ds- (BGK: both into slot 0 pattern [s+, 0]) to dtau_part[2 g], [2 g + 1]
- * for workgroup g; the caller zeroes dtau_part once and sums it at the end. */
+ * for workgroup g; the caller zeroes dtau_part once and sums it at the end.  dtau_part may
+ * be NULL: the rate sensitivities are then not computed (a rollout that does not
+ * differentiate tau). */
 int pylbm_hip_adjoint_step(const pylbm_params* p, const void* f_t, const void* lam_in,
                            void* lam_out, float* dF, double* dtau_part, void* stream);
 
@@ -234,7 +236,7 @@ int pylbm_hip_adjoint_step(const pylbm_params* p, const void* f_t, const void* l
  * entry) and lam_b; *which = 0 if the final adjoint state is in lam_a, 1 if in lam_b.
  * Pairs of steps run in place in lam_a (an odd last step goes to lam_b); the result equals
  * the ping-pong of pylbm_hip_adjoint_step bit for bit.
- * dF and dtau_part accumulate as in pylbm_hip_adjoint_step. */
+ * dF and dtau_part accumulate as in pylbm_hip_adjoint_step (dtau_part NULL: none). */
 int pylbm_hip_adjoint_run(const pylbm_params* p, const void* const* states, int64_t nsteps,
                           void* lam_a, void* lam_b, float* dF, double* dtau_part, int* which,
                           void* stream);

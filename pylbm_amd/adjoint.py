@@ -209,8 +209,13 @@ class _RolloutFn(torch.autograd.Function):
         lam = [torch.empty(q * N, dtype=ldt, device=eng.dev) for _ in range(2)]
         dF = torch.zeros(d * N, dtype=torch.float32, device=eng.dev) if eng.ff is not None \
             else None
-        nw = int(_lib.load().pylbm_hip_adjoint_workspace(eng.ref))
-        part = torch.zeros(max(nw, 2), dtype=torch.float64, device=eng.dev)
+        need_F, need_tau, need_u, need_rho = ctx.needs
+        # the rate sensitivities cost the kernels a per-population sum and a workgroup
+        # reduction: only when dL/dtau is wanted
+        part = None
+        if need_tau:
+            nw = int(_lib.load().pylbm_hip_adjoint_workspace(eng.ref))
+            part = torch.zeros(max(nw, 2), dtype=torch.float64, device=eng.dev)
         _lib.call("pylbm_hip_adjoint_seed", eng.ref, _p(ctx.fT), _p(G), _p(lam[0]), _p(dF),
                   _s())
         cur = 0  # index of the lam buffer holding the current adjoint state
@@ -261,7 +266,6 @@ class _RolloutFn(torch.autograd.Function):
                 ctx.last = None
         main.wait_stream(side)  # the segment buffers are freed after the side stream's work
         lam0 = lam[cur]
-        need_F, need_tau, need_u, need_rho = ctx.needs
         du0 = drho0 = None
         if need_u or need_rho:
             du = torch.empty((d,) + eng.p.array_shape, dtype=torch.float64, device=eng.dev) \

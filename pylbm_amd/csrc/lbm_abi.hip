@@ -453,7 +453,7 @@ int pylbm_hip_adjoint_seed(const pylbm_params* p, const void* f_T, const float* 
 int pylbm_hip_adjoint_step(const pylbm_params* p, const void* f_t, const void* lam_in,
                            void* lam_out, float* dF, double* dtau_part, void* stream) {
   if (int e = check_adjoint(p, "pylbm_hip_adjoint_step")) return e;
-  if (!f_t || !lam_in || !lam_out || !dtau_part)
+  if (!f_t || !lam_in || !lam_out)
     return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_step: NULL buffer");
   if (lam_in == lam_out)
     return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_step: lam_in and lam_out must differ");
@@ -472,7 +472,7 @@ int pylbm_hip_adjoint_run(const pylbm_params* p, const void* const* states, int6
                           void* stream) {
   if (int e = check_adjoint(p, "pylbm_hip_adjoint_run")) return e;
   if (nsteps < 0 || !which) return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_run: bad arguments");
-  if (nsteps > 0 && (!states || !lam_a || !lam_b || lam_a == lam_b || !dtau_part))
+  if (nsteps > 0 && (!states || !lam_a || !lam_b || lam_a == lam_b))
     return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_run: NULL or aliased buffer");
   if (p->force_field && !dF)
     return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_run: a force field needs dF");

@@ -28,26 +28,16 @@ namespace plbm {
 
 namespace {
 
-// Per-node forward quantities and the transposed collision Jacobian.  h_i = g_i - w_i is
-// the deviation of the pulled population from its rest weight (exact in fp32 for g near
-// w), and every difference the Jacobian needs (g - feq, feq - w) is formed from
-// deviations, so the small non-equilibrium part is not lost to cancellation.  With
-// feq_i = w_i rho (1 + poly_i), poly_i = 3 cu_i + 4.5 cu_i^2 - 1.5 u^2:
-//   A    = sum_i lam'_i feq_i / rho = sum_i lam'_i w_i (1 + poly_i)
-//   B_a  = 3 rho (sum_i lam'_i w_i c_ia (1 + 3 cu_i) - u_a sum_i lam'_i w_i)
-//          [+ 9 sum_i lam''_i w_i cf_i c_ia - 3 F_a sum_i lam''_i w_i   with a force]
-//   mu_k = lam_k - lam'_k + (A - B.u / rho) + sum_a c_ka B_a / rho
-// FORCE = false drops the Guo terms at compile time.
-template <class L, int COLL, bool FORCE>
-__device__ __forceinline__ void adjoint_node(const float (&h)[L::Q], const float F[3],
-                                             const float (&lam)[L::Q], float sp, float sm,
-                                             float (&mu)[L::Q], float dF[3], float& dsp,
-                                             float& dsm) {
-  // multiply-adds fuse within a statement only, so the two-buffer and the in-place (AA)
-  // adjoint kernels round every node identically whatever their store pattern
+// Per-node forward quantities: the moments of the pulled deviations h_i = g_i - w_i (exact
+// in fp32 for g near w), summed in the order of the forward collision.
+template <class L, bool FORCE>
+__device__ __forceinline__ void pulled_moments(const float (&h)[L::Q], const float F[3],
+                                               float& drho, float& rho, float& inv,
+                                               float (&u)[3]) {
 #pragma clang fp contract(on)
   constexpr int Q = L::Q, D = L::D;
-  float drho = 0.f, j[3] = {0.f, 0.f, 0.f};
+  float j[3] = {0.f, 0.f, 0.f};
+  drho = 0.f;
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     drho += h[i];
@@ -55,86 +45,151 @@ __device__ __forceinline__ void adjoint_node(const float (&h)[L::Q], const float
     for (int a = 0; a < D; ++a)
       if (L::c(i, a)) j[a] += L::c(i, a) * h[i];
   }
-  const float rho = 1.0f + drho;
-  const float inv = __builtin_amdgcn_rcpf(rho);
-  float u[3] = {0.f, 0.f, 0.f};
+  rho = 1.0f + drho;
+  inv = __builtin_amdgcn_rcpf(rho);
+  u[0] = u[1] = u[2] = 0.f;
 #pragma unroll
   for (int a = 0; a < D; ++a) u[a] = (FORCE ? j[a] + 0.5f * F[a] : j[a]) * inv;
+}
+
+// The transposed collision Jacobian mu = J_C^T lam at one node, from rho, u (and F), in
+// moment form.  lam+- = P+- lam, lam' = s+ lam+ + s- lam-, lam'' = g+ lam+ + g- lam-
+// (BGK: s+ = s- = omega).  With the weighted moments of lam over opposite pairs
+// (s_p = lam_i + lam_o, d_p = lam_i - lam_o):
+//   m0 = sum w lam,  m1_a = sum w c_a lam,  M2_ab = sum w c_a c_b lam
+// (m0 and M2 see lam+ only, m1 lam- only), the sums of the adjoint become
+//   A    = sum lam' w (1 + poly) = s+ (m0 (1 - 1.5 u^2) + 4.5 u.M2.u) + 3 s- u.m1
+//   B_a  = 3 rho (s- m1_a + 3 s+ (M2 u)_a - u_a s+ m0)
+//          [+ 9 g+ (M2 F)_a - 3 F_a g+ m0                          with a force]
+//   mu_k = lam_k - lam'_k + (A - B.u / rho) + c_k.B / rho
+//   dF_a = B_a / (2 rho) + 3 g- m1_a + 9 g+ (M2 u)_a - 3 u_a g+ m0   (force only)
+// poly = 3 cu + 4.5 cu^2 - 1.5 u^2.  About a third of the per-population form's vector
+// instructions: the 10 moments take 63 multiply-adds for D3Q19, the pushes two per pair.
+template <class L, int COLL, bool FORCE>
+__device__ __forceinline__ void adjoint_core(float rho, float inv, const float (&u)[3],
+                                             const float F[3], const float (&lam)[L::Q],
+                                             float sp, float sm, float (&mu)[L::Q],
+                                             float dF[3]) {
+  // multiply-adds fuse within a statement only, so every adjoint kernel (two-buffer,
+  // in place, two steps per launch) rounds every node identically
+#pragma clang fp contract(on)
+  constexpr int Q = L::Q, D = L::D, NP = (Q - 1) / 2;
+  const float s_p = sp, s_m = (COLL == PYLBM_BGK) ? sp : sm;
+  float ps[NP], pd[NP];
+  float m0 = (float)L::w(0) * lam[0];
+  float m1[3] = {0.f, 0.f, 0.f};
+  float M2[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    const int i = pair_lo<L>(k), o = L::opp(i);
+    ps[k] = lam[i] + lam[o];
+    pd[k] = lam[i] - lam[o];
+    const float w = (float)L::w(i);
+    m0 += w * ps[k];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      if (L::c(i, a) == 0) continue;
+      m1[a] += (L::c(i, a) > 0 ? w : -w) * pd[k];
+#pragma unroll
+      for (int b = a; b < D; ++b)
+        if (L::c(i, b) != 0) M2[a][b] += (L::c(i, a) * L::c(i, b) > 0 ? w : -w) * ps[k];
+    }
+  }
+  float M2u[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int b = 0; b < D; ++b) M2u[a] += (a <= b ? M2[a][b] : M2[b][a]) * u[b];
+  float uM2u = 0.f, um1 = 0.f;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    uM2u += u[a] * M2u[a];
+    um1 += u[a] * m1[a];
+  }
+  const float usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  const float A = s_p * (m0 * (1.0f - 1.5f * usq) + 4.5f * uM2u) + (3.0f * s_m) * um1;
+  const float P = s_p * m0;
+  const float gp = 1.0f - 0.5f * s_p, gm = 1.0f - 0.5f * s_m;
+  float Bn[3] = {0.f, 0.f, 0.f}, Bu = 0.f;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    float B = 3.0f * rho * ((s_m * m1[a] + (3.0f * s_p) * M2u[a]) - u[a] * P);
+    if (FORCE) {
+      float M2F = 0.f;
+#pragma unroll
+      for (int b = 0; b < D; ++b) M2F += (a <= b ? M2[a][b] : M2[b][a]) * F[b];
+      B += 9.0f * (gp * M2F) - 3.0f * F[a] * (gp * m0);
+    }
+    Bn[a] = B * inv;
+    Bu += Bn[a] * u[a];
+  }
+  const float K0 = A - Bu;
+  const float hp = 0.5f * (1.0f - s_p), hm = 0.5f * (1.0f - s_m);
+  mu[0] = (1.0f - s_p) * lam[0] + K0;
+#pragma unroll
+  for (int k = 0; k < NP; ++k) {
+    const int i = pair_lo<L>(k), o = L::opp(i);
+    float cB = 0.f;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      if (L::c(i, a) > 0) cB += Bn[a];
+      else if (L::c(i, a) < 0) cB -= Bn[a];
+    }
+    const float base = hp * ps[k] + K0;
+    const float diff = hm * pd[k] + cB;
+    mu[i] = base + diff;
+    mu[o] = base - diff;
+  }
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+    dF[a] = FORCE ? 0.5f * Bn[a] + ((3.0f * gm) * m1[a] + (9.0f * gp) * M2u[a]) -
+                        3.0f * u[a] * (gp * m0)
+                  : 0.0f;
+}
+
+// The rate sensitivities of one node: dL/ds+- += -sum_i lam+-_i r_i with
+// r_i = g_i - feq_i + S_i / 2 (BGK: dL/domega into dsp).  r is the small non-equilibrium
+// part; it is formed per population from deviations (h_i - (feq_i - w_i)) before it meets
+// lam, so it keeps its digits (summing lam h and lam (feq - w) separately would cancel).
+template <class L, int COLL, bool FORCE>
+__device__ __forceinline__ void rate_sensitivity(const float (&h)[L::Q], float drho,
+                                                 float rho, const float (&u)[3],
+                                                 const float F[3], const float (&lam)[L::Q],
+                                                 float& dsp, float& dsm) {
+#pragma clang fp contract(on)
+  constexpr int Q = L::Q;
   const float usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
   const float uf = FORCE ? u[0] * F[0] + u[1] * F[1] + u[2] * F[2] : 0.0f;
-  const float gp = 1.0f - 0.5f * sp, gm = 1.0f - 0.5f * sm;
-
-  float A = 0.f, P = 0.f, Pp = 0.f;            // sum lam' w (1 + poly), sum lam' w, sum lam'' w
-  float Bc[3] = {0.f, 0.f, 0.f};              // sum lam' w c_a (1 + 3 cu)
-  float Sc[3] = {0.f, 0.f, 0.f};              // sum lam'' w c_a cf        (force)
-  float Dc[3] = {0.f, 0.f, 0.f};              // sum lam'' w c_a (3 + 9 cu) (force)
-  dsp = 0.f;
-  dsm = 0.f;
-  float lp_[Q];
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     const int o = L::opp(i);
-    const float lplus = 0.5f * (lam[i] + lam[o]);
-    const float lminus = 0.5f * (lam[i] - lam[o]);
-    const float lp = (COLL == PYLBM_BGK) ? sp * lam[i] : sp * lplus + sm * lminus;
-    lp_[i] = lp;
     const float w = (float)L::w(i);
     const float cu = L::c(i, 0) * u[0] + L::c(i, 1) * u[1] + L::c(i, 2) * u[2];
     const float poly = cu * (3.0f + 4.5f * cu) - 1.5f * usq;
-    const float lw = lp * w;
-    A += lw * (1.0f + poly);
-    P += lw;
-    const float t = lw * (1.0f + 3.0f * cu);
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-      if (L::c(i, a) > 0) Bc[a] += t;
-      else if (L::c(i, a) < 0) Bc[a] -= t;
-    }
     float S = 0.0f;
     if (FORCE) {
-      const float lpp = (COLL == PYLBM_BGK) ? gp * lam[i] : gp * lplus + gm * lminus;
-      const float lppw = lpp * w;
       const float cf = L::c(i, 0) * F[0] + L::c(i, 1) * F[1] + L::c(i, 2) * F[2];
       S = w * (3.0f * (cf - uf) + 9.0f * cu * cf);
-      Pp += lppw;
-      const float tc = lppw * cf, td = lppw * (3.0f + 9.0f * cu);
-#pragma unroll
-      for (int a = 0; a < D; ++a) {
-        if (L::c(i, a) > 0) { Sc[a] += tc; Dc[a] += td; }
-        else if (L::c(i, a) < 0) { Sc[a] -= tc; Dc[a] -= td; }
-      }
     }
     const float r = (h[i] - w * (drho + rho * poly)) + 0.5f * S;  // g - feq + S/2
     if (COLL == PYLBM_BGK) {
       dsp -= lam[i] * r;
     } else {
-      dsp -= lplus * r;
-      dsm -= lminus * r;
+      dsp -= 0.5f * (lam[i] + lam[o]) * r;
+      dsm -= 0.5f * (lam[i] - lam[o]) * r;
     }
   }
-  float B[3], Bn[3];
-  float Bu = 0.f;
-#pragma unroll
-  for (int a = 0; a < D; ++a) {
-    B[a] = 3.0f * rho * (Bc[a] - u[a] * P);
-    if (FORCE) B[a] += 9.0f * Sc[a] - 3.0f * F[a] * Pp;
-    Bn[a] = B[a] * inv;
-    Bu += Bn[a] * u[a];
-  }
-  const float K0 = A - Bu;
-#pragma unroll
-  for (int k = 0; k < Q; ++k) {
-    float m = lam[k] - lp_[k] + K0;
-#pragma unroll
-    for (int a = 0; a < D; ++a) {
-      if (L::c(k, a) > 0) m += Bn[a];
-      else if (L::c(k, a) < 0) m -= Bn[a];
-    }
-    mu[k] = m;
-  }
-#pragma unroll
-  for (int a = 0; a < D; ++a)
-    dF[a] = FORCE ? 0.5f * Bn[a] + (Dc[a] - 3.0f * u[a] * Pp) : 0.0f;
+}
+
+// The adjoint state element of an fp32 value: fp32 as it is, or rounded to fp16.  The empty
+// asm keeps the value an fp32 register first, as the source writes it: otherwise the
+// compiler may fold the last multiply-add and the conversion into one mixed-precision
+// instruction that rounds once to fp16 (v_fma_mixlo_f16), in some kernels and not in
+// others, and the in-place and two-buffer adjoints would differ in the last fp16 bit.
+template <class LT>
+__device__ __forceinline__ LT to_lam(float v) {
+  if constexpr (sizeof(LT) == 2) asm volatile("" : "+v"(v));
+  return (LT)v;
 }
 
 __device__ __forceinline__ float wave_sum_f(float v) {
@@ -157,7 +212,7 @@ __device__ __forceinline__ float wave_sum_f(float v) {
 //                  leaves the state natural again.
 // In both AA steps a node writes exactly the slots it read, so the update in place has no
 // race; a pair of them equals two two-buffer steps bit for bit.
-template <class L, int COLL, int ST, int FM, int ADDR, class LT, int SP, int PAT>
+template <class L, int COLL, int ST, int FM, int ADDR, class LT, int SP, int PAT, bool RS>
 __global__ void __launch_bounds__(kBlock)
 k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f, const LT* lam_in,
           LT* lam_out, float* __restrict__ dF, double* __restrict__ part) {
@@ -223,15 +278,17 @@ k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f, const LT* 
       for (int d = 0; d < L::D; ++d) dF0[d] = ld_pl<float, ADDR>(dFb, fplane, d, 0, f0);
     }
     gather_finish<L, ST, PYLBM_AB, ADDR>(l, fb, N * TS, t, lid, h);
-    float mu[Q], dFn[3];
-    adjoint_node<L, COLL, FM != 0>(h, F, lam, a.r0, a.r1, mu, dFn, dsp, dsm);
+    float mu[Q], dFn[3], drho, rho, inv, u[3];
+    pulled_moments<L, FM != 0>(h, F, drho, rho, inv, u);
+    adjoint_core<L, COLL, FM != 0>(rho, inv, u, F, lam, a.r0, a.r1, mu, dFn);
+    if (RS) rate_sensitivity<L, COLL, FM != 0>(h, drho, rho, u, F, lam, dsp, dsm);
     // push mu_i back along the link it was pulled over: to (i, x - c_i), or to the own
     // slot opp(i) on a wall link.  Waves with no lane on an x wall (every wave of a box
     // periodic in x) take a loop without per-lane branches; the per-lane wall choice costs
     // each direction a divergent branch and its exec-mask bookkeeping on the scalar unit.
     if constexpr (PAT == PYLBM_AA_EVEN) {
 #pragma unroll
-      for (int i = 0; i < Q; ++i) st_pl<SP, ADDR>(lob, lplane, L::opp(i), 0, q0, (LT)mu[i]);
+      for (int i = 0; i < Q; ++i) st_pl<SP, ADDR>(lob, lplane, L::opp(i), 0, q0, to_lam<LT>(mu[i]));
     } else if (__ballot(l.wall_lo || l.wall_hi) == 0) {
       // a y/z wall (uniform) selects the own slot opp(i) by value, without a branch
 #pragma unroll
@@ -240,12 +297,12 @@ k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f, const LT* 
         const RowLink r = row_link<L>(a, l.y, l.z, i, -1);
         const bool own = i == 0 || r.wall;
         st_pl<SP, ADDR>(lob, lplane, own ? L::opp(i) : i, own ? 0 : r.delta * (ptrdiff_t)LS,
-                        own ? q0 : (cx > 0 ? qm : (cx < 0 ? qp : q0)), (LT)mu[i]);
+                        own ? q0 : (cx > 0 ? qm : (cx < 0 ? qp : q0)), to_lam<LT>(mu[i]));
       }
     } else {
 #pragma unroll
       for (int i = 0; i < Q; ++i) {
-        const LT m = (LT)mu[i];
+        const LT m = to_lam<LT>(mu[i]);
         if (i == 0) {
           st_pl<SP, ADDR>(lob, lplane, 0, 0, q0, m);
           continue;
@@ -268,6 +325,7 @@ k_adjoint(const KArgs a, const typename Codec<ST>::T* __restrict__ f, const LT* 
       for (int d = 0; d < L::D; ++d) st_pl<0, ADDR>(dFb, fplane, d, 0, f0, dF0[d] + dFn[d]);
     }
   }
+  if (!RS) return;
   // per-workgroup partial sums of dL/ds+ and dL/ds- (deterministic: one slot per group)
   __shared__ float red[2][kBlock / 64];
   dsp = wave_sum_f(dsp);
@@ -366,9 +424,9 @@ k_adjoint_init(int N_, const double* __restrict__ rho0, const double* __restrict
   if (drho) drho[n] = gr;
 }
 
-template <class L, int COLL, int ST, class LT, int PAT>
-int adj_step_tp(const KArgs& a, bool field, const void* f, const void* lin, void* lout,
-                float* dF, double* part, hipStream_t s) {
+template <class L, int COLL, int ST, class LT, int PAT, bool RS>
+int adj_step_tpr(const KArgs& a, bool field, const void* f, const void* lin, void* lout,
+                 float* dF, double* part, hipStream_t s) {
   using T = typename Codec<ST>::T;
   const dim3 grid((unsigned)a.nblk), block(kBlock);
   const bool uniform = a.fx != 0.0f || a.fy != 0.0f || a.fz != 0.0f;
@@ -380,29 +438,33 @@ int adj_step_tp(const KArgs& a, bool field, const void* f, const void* lin, void
   const bool small = (uint64_t)L::Q * (uint64_t)a.N * 4u < ((uint64_t)1 << 32);
   // SP: temporal adjoint-state stores when the forward state is resident in the Infinity
   // Cache (the next adjoint step re-reads them), non-temporal otherwise
+#define PLBM_ADJ(FMv, ADDRv, SPv)                                                          \
+  k_adjoint<L, COLL, ST, FMv, ADDRv, LT, SPv, PAT, RS><<<grid, block, 0, s>>>(a, fs, li, lo, \
+                                                                            dF, part)
   if (small && a.resident) {
-    if (field)
-      k_adjoint<L, COLL, ST, 2, 1, LT, 0, PAT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
-    else if (uniform)
-      k_adjoint<L, COLL, ST, 1, 1, LT, 0, PAT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
-    else
-      k_adjoint<L, COLL, ST, 0, 1, LT, 0, PAT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
+    if (field) PLBM_ADJ(2, 1, 0);
+    else if (uniform) PLBM_ADJ(1, 1, 0);
+    else PLBM_ADJ(0, 1, 0);
   } else if (small) {
-    if (field)
-      k_adjoint<L, COLL, ST, 2, 1, LT, 2, PAT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
-    else if (uniform)
-      k_adjoint<L, COLL, ST, 1, 1, LT, 2, PAT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
-    else
-      k_adjoint<L, COLL, ST, 0, 1, LT, 2, PAT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
+    if (field) PLBM_ADJ(2, 1, 2);
+    else if (uniform) PLBM_ADJ(1, 1, 2);
+    else PLBM_ADJ(0, 1, 2);
   } else {
-    if (field)
-      k_adjoint<L, COLL, ST, 2, 0, LT, 2, PAT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
-    else if (uniform)
-      k_adjoint<L, COLL, ST, 1, 0, LT, 2, PAT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
-    else
-      k_adjoint<L, COLL, ST, 0, 0, LT, 2, PAT><<<grid, block, 0, s>>>(a, fs, li, lo, dF, part);
+    if (field) PLBM_ADJ(2, 0, 2);
+    else if (uniform) PLBM_ADJ(1, 0, 2);
+    else PLBM_ADJ(0, 0, 2);
   }
+#undef PLBM_ADJ
   return (int)hipGetLastError();
+}
+
+// part == nullptr: no rate sensitivities (rollouts that do not differentiate tau skip
+// their per-population sums and the workgroup reduction)
+template <class L, int COLL, int ST, class LT, int PAT>
+int adj_step_tp(const KArgs& a, bool field, const void* f, const void* lin, void* lout,
+                float* dF, double* part, hipStream_t s) {
+  if (part) return adj_step_tpr<L, COLL, ST, LT, PAT, true>(a, field, f, lin, lout, dF, part, s);
+  return adj_step_tpr<L, COLL, ST, LT, PAT, false>(a, field, f, lin, lout, dF, part, s);
 }
 
 template <class L, int COLL, int ST, class LT>

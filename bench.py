@@ -25,6 +25,7 @@ only one step kernel (the headline by default) and the copy kernel (for rocprofv
 from __future__ import annotations
 
 import argparse
+import datetime
 import json
 import os
 import platform
@@ -504,6 +505,7 @@ def main() -> int:
                              f"calibrated step count, each >= {a.min_time} s (median; "
                              "pylbm/benchmark.py:159-180)"},
         "machine": gpu_state(),
+        "date": datetime.datetime.now().astimezone().isoformat(timespec="seconds"),
         "variants": variants,
         "fwd_adjoint": adjoint,
         "device": dev,

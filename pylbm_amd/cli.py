@@ -7,7 +7,9 @@ for the ``pylbm`` console script runs the gfx950 kernels by changing the program
   device;
 * ``validate``, ``bench`` -- the validation suite and the benchmark plans of this package;
 * ``backends`` -- every registered entry, whether it runs here, its storage and layout;
-* ``report`` -- Markdown tables and README blocks from ``bench`` / ``validate`` records.
+* ``report`` -- Markdown tables and README blocks from ``bench`` / ``validate`` records;
+  with ``--tex`` the paper's ``results.tex`` macros and with ``--figures`` its figures
+  (the reference's figures/make_figures.py).
 
 The reference's ``kernels`` command prints the source its code generator writes for a DSL;
 the hip kernels are fixed HIP templates, so there is no such command.
@@ -158,9 +160,30 @@ def _cmd_report(a: argparse.Namespace) -> int:
     from . import report
 
     records = {k: report.load(v) if v else None
-               for k, v in (("bench", a.bench), ("validation", a.validation))}
+               for k, v in (("bench", a.bench), ("validation", a.validation),
+                            ("counterparts", a.counterparts))}
     if not any(records.values()):
-        raise SystemExit("report: give --bench and/or --validation")
+        raise SystemExit("report: give --bench, --validation and/or --counterparts")
+    if a.tex or a.figures:
+        from . import paper
+
+        if a.tex:
+            src = Path(a.bench).name if a.bench else "none"
+            Path(a.tex).write_text(paper.results_tex(records["bench"], records["validation"],
+                                                     records["counterparts"], source=src))
+            print(f"wrote {a.tex}")
+        if a.figures:
+            vort = None
+            if a.vorticity:
+                import numpy as np
+
+                with np.load(a.vorticity) as z:
+                    vort = {"vorticity": z["vorticity"], "solid": z["solid"]}
+            for p in paper.figures(a.figures, records["bench"], records["validation"],
+                                   records["counterparts"], vort):
+                print(f"wrote {p}")
+        if not (a.out or a.readme):
+            return 0
     if a.readme:
         path = Path(a.readme)
         blocks = report.blocks_for(records["bench"], records["validation"])
@@ -211,6 +234,12 @@ def _add_report(sp: argparse.ArgumentParser) -> None:
     sp.add_argument("--out", default=None, help="write the report here (default: stdout)")
     sp.add_argument("--readme", default=None,
                     help="refresh the <!-- pylbm_amd:NAME --> blocks of this Markdown file")
+    sp.add_argument("--counterparts", default=None, help="head-to-head record (JSON)")
+    sp.add_argument("--tex", default=None, help="write the paper's results.tex macros here")
+    sp.add_argument("--figures", default=None,
+                    help="draw the paper's figures (pdf and png) into this folder")
+    sp.add_argument("--vorticity", default=None,
+                    help="validate --vorticity-out file for the cylinder figure")
 
 
 #: command -> (help, argument builder, handler)
@@ -219,7 +248,8 @@ COMMANDS = {
     "run": ("run one case on the device and report its throughput", _add_run, _cmd_run),
     "validate": ("run the validation suite", _add_validate, _cmd_validate),
     "bench": ("benchmark every hip variant (cavity, Re = 1000)", _add_bench, _cmd_bench),
-    "report": ("Markdown tables from bench / validate records", _add_report, _cmd_report),
+    "report": ("Markdown tables, paper macros and figures from bench / validate records",
+               _add_report, _cmd_report),
 }
 
 

@@ -55,13 +55,14 @@ def from_bench_line(line: dict) -> dict:
             "backend": "hip", "variant": v["variant"], "lattice": v["lattice"],
             "collision": v["collision"], "size": list(v["size"]),
             "storage": v.get("storage"), "metric": "MLUPS", "value": v["MLUPS"],
-            "median": v["MLUPS"], "min": v["MLUPS"], "max": v["MLUPS"],
-            "repeats": 1, "steps_per_repeat": v.get("steps"),
+            "median": v["MLUPS"], "min": v.get("MLUPS_min", v["MLUPS"]),
+            "max": v.get("MLUPS_max", v["MLUPS"]), "repeats": v.get("repeats", 1),
+            "steps_per_repeat": v.get("steps"),
             "bytes_per_unit": v["bytes_per_update"], "effective_GBs": gbs,
             "pct_copy_bw": 100.0 * gbs / copy_bw if copy_bw else None,
             "pct_spec_bw": 100.0 * gbs / spec,
             "workload": v.get("workload", "")})
-    return {"machine": {"gpu": line.get("device", "GPU")},
+    return {"machine": {"gpu": line.get("device", "GPU"), "date": line.get("date", "")},
             "ceiling": {"copy_bandwidth_GBs": copy_bw, "spec_bandwidth_GBs": spec},
             "plan": "bench.py", "provisional": False, "runs": runs}
 

@@ -341,13 +341,13 @@ def main() -> int:
     import torch
 
     from pylbm_amd import _lib
-    from pylbm_amd.benchmark import SPEC_BANDWIDTH_GBS, copy_bandwidth, gpu_state
+    from pylbm_amd.benchmark import SPEC_BANDWIDTH_GBS, copy_bandwidth, device_name, gpu_state
 
     if not torch.cuda.is_available():
         print("bench.py needs a HIP GPU", file=sys.stderr)
         return 1
     _lib.load()
-    dev = torch.cuda.get_device_name(0)
+    dev = device_name()
     log(f"device {dev}")
     if a.adjoint_only:
         for case in ADJOINT_CASES:

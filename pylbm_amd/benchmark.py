@@ -129,6 +129,23 @@ def gpu_state() -> dict:
         return {}
 
 
+def device_name() -> str:
+    """Product name of GPU 0: the ASIC's market name from ``amdsmi`` when it reads one
+    (torch reports the generic "AMD Radeon Graphics" on some drivers), else torch's name."""
+    try:
+        import amdsmi
+
+        if "h" not in _SMI:
+            amdsmi.amdsmi_init()
+            _SMI["h"] = amdsmi.amdsmi_get_processor_handles()[0]
+        name = str(amdsmi.amdsmi_get_gpu_asic_info(_SMI["h"]).get("market_name", "")).strip()
+        if name and name.upper() != "N/A":
+            return name
+    except Exception:  # noqa: BLE001 - fall back to torch's name
+        pass
+    return _torch().cuda.get_device_name(0)
+
+
 def state_delta(s0: dict, s1: dict) -> dict:
     """Snapshot after a window, plus the growth of each accumulated limiter counter."""
     out = dict(s1)
@@ -347,7 +364,7 @@ def run(plan_name: str = "quick", out: str | None = None, log: Any = print) -> d
             runs.append(r)
     date = datetime.datetime.now().astimezone().isoformat(timespec="seconds")
     result = {"machine": {"python": platform.python_version(), "date": date,
-                          "torch": torch.__version__, "gpu": torch.cuda.get_device_name(0)},
+                          "torch": torch.__version__, "gpu": device_name()},
               "ceiling": {"copy_bandwidth_GBs": cbw, "spec_bandwidth_GBs": SPEC_BANDWIDTH_GBS,
                           "fp32_tflops": None, **ceil},
               "plan": plan_name, "provisional": False,

@@ -450,7 +450,15 @@ def _save(fig, out_dir: Path, name: str, formats: Iterable[str]) -> list[Path]:
     return paths
 
 
+#: hip rungs share a backend, so their figures colour by variant
+HIP_COLOURS = {"gather": "#8c564b", "fused": "#d62728", "fused-f32s": "#ff9896",
+               "fused-aa": "#ff7f0e", "fused-fp16": "#1f77b4", "fused-bf16": "#9467bd",
+               "fused-aa-fp16": "#2ca02c", "fused-aa-bf16": "#17becf"}
+
+
 def _colour(key: Rung) -> str:
+    if key[0] == "hip":
+        return HIP_COLOURS.get(key[1], COLOURS["hip"])
     return COLOURS.get(key[0], "#444444")
 
 

@@ -268,10 +268,12 @@ def cpu_baseline(n: int, steps: int) -> dict:
             "seconds": dt}
 
 
-def north_star_ratios(variants: list, adjoint: list, head: dict, copy_gbs: float) -> dict:
+def north_star_ratios(variants: list, adjoint: list, head: dict, copy_gbs: float,
+                      planes: dict | None = None) -> dict:
     """The north-star targets computed from this run's own legs: fp32 fraction of the copy
     ceiling, 16-bit over fp32 MLUPS per collision and pattern (legs of the same protocol),
-    and forward+adjoint over forward-only."""
+    forward+adjoint over forward-only, and each two-buffer leg's bandwidth as a fraction
+    of the plane stream of its own shape (``planes``, from plane_stream_bandwidth)."""
     def leg(variant, coll, lattice="D2Q9"):
         for r in variants:
             if (r["variant"] == variant and r["collision"] == coll and r["lattice"] == lattice
@@ -296,6 +298,23 @@ def north_star_ratios(variants: list, adjoint: list, head: dict, copy_gbs: float
                                                     leg("fused-aa", "trt", "D3Q19"))
     for r in adjoint:
         out[f"fwd_adjoint_ratio_{r['case']}"] = r["ratio"]
+    if planes:
+        def gbs(variant, coll, lattice):
+            for r in variants:
+                if (r["variant"] == variant and r["collision"] == coll
+                        and r["lattice"] == lattice and "workload" not in r
+                        and r.get("protocol") != "headline"):
+                    return r["GBs"]
+            return None
+
+        for key, shape, g in (
+                ("d2q9_fp32_bgk", "d2q9_f32", head["GBs"]),
+                ("d2q9_fp16_bgk", "d2q9_16", gbs("fused-fp16", "bgk", "D2Q9")),
+                ("d3q19_fp32_trt", "d3q19_f32", gbs("fused", "trt", "D3Q19")),
+                ("d3q19_fp16_trt", "d3q19_16", gbs("fused-fp16", "trt", "D3Q19")),
+                ("d3q19_bf16_trt", "d3q19_16", gbs("fused-bf16", "trt", "D3Q19"))):
+            ref = planes.get(f"{shape}_GBs")
+            out[f"{key}_frac_of_plane_stream"] = ratio(g, ref)
     return out
 
 
@@ -314,7 +333,7 @@ def main() -> int:
     ap.add_argument("--quick", action="store_true", help="headline + copy ceiling only")
     ap.add_argument("--profile", nargs="?", const="d2q9", default=None,
                     choices=["d2q9", "d3q19", "d3q19-fp16", "d3q19-aa", "d3q19-aa-fp16",
-                             "geometry-d2q9", "geometry-d3q19", "copy"],
+                             "geometry-d2q9", "geometry-d3q19", "copy", "planes"],
                     help="only one step kernel and the copy kernel, for rocprofv3: the headline "
                          "(d2q9, default), D3Q19 512^3 TRT fp32 / fp16, two-buffer or AA, or "
                          "the geometry kernel (fp16) on the cylinder-array channel 4096^2 / "
@@ -341,7 +360,8 @@ def main() -> int:
     import torch
 
     from pylbm_amd import _lib
-    from pylbm_amd.benchmark import SPEC_BANDWIDTH_GBS, copy_bandwidth, device_name, gpu_state
+    from pylbm_amd.benchmark import (SPEC_BANDWIDTH_GBS, copy_bandwidth, device_name,
+                                     gpu_state, plane_stream_bandwidth)
 
     if not torch.cuda.is_available():
         print("bench.py needs a HIP GPU", file=sys.stderr)
@@ -371,6 +391,23 @@ def main() -> int:
                               y.data_ptr(), nbytes, st)
         torch.cuda.synchronize()
         log(f"profile: {a.steps} launches of each stream probe over {nbytes} B done")
+        return 0
+    if a.profile == "planes":
+        # the plane-stream probe at the state shapes of D2Q9 n2^2 and D3Q19 n3^3 (fp32 and
+        # 16-bit), a.steps launches each, in that order
+        st = int(torch.cuda.current_stream().cuda_stream)
+        for planes, eb, n_el in ((9, 4, a.n2 ** 2), (9, 2, a.n2 ** 2), (19, 4, a.n3 ** 3),
+                                 (19, 2, a.n3 ** 3)):
+            x = torch.empty(planes * n_el * eb // 4, dtype=torch.float32,
+                            device="cuda").fill_(1.0)
+            y = torch.empty_like(x)
+            for _ in range(a.steps):
+                _lib.call("pylbm_hip_plane_probe", planes, eb, x.data_ptr(), y.data_ptr(),
+                          n_el * eb, st)
+            torch.cuda.synchronize()
+            del x, y
+            torch.cuda.empty_cache()
+            log(f"profile: {a.steps} plane probes, {planes} planes x {n_el} x {eb} B done")
         return 0
     rho, u = shear_wave_init(n, SEED)
 
@@ -421,6 +458,16 @@ def main() -> int:
     same = copy_bandwidth(max(1, 9 * a.n2 * a.n2 * 4 // 2**20), repeats=20)
     log(f"copy at the headline's size ({same['mib']} MiB each way): "
         f"{same['copy_bandwidth_GBs']:.0f} GB/s ({same['copy_best']})")
+    # the S-plane stream at the state shapes of the legs: the ceiling of their access
+    # pattern (S planes read, S written per element), beside the two-stream copy
+    planes = None
+    if not a.quick:
+        n3 = a.n3
+        planes = plane_stream_bandwidth(
+            {"d2q9_f32": (9, 4, n * n), "d2q9_16": (9, 2, n * n),
+             "d3q19_f32": (19, 4, n3 ** 3), "d3q19_16": (19, 2, n3 ** 3)})
+        log("plane streams: " + ", ".join(f"{k[:-4]} {v:.0f} GB/s" for k, v in planes.items()
+                                          if k.endswith("_GBs") and "_max" not in k))
 
     proto = {"min_time": a.min_time, "repeats": a.repeats, "warm_time": a.warm_time}
     head = run_variant("fused", problem_2d("bgk", n, rho, u), (rho, u), proto,
@@ -463,7 +510,7 @@ def main() -> int:
             log(f"  fwd+adjoint {case}: {r['fwd_adjoint_MLUPS']:.0f} MLUPS vs forward "
                 f"{r['forward_MLUPS']:.0f} ({100 * r['ratio']:.1f}%)")
 
-    targets = north_star_ratios(variants, adjoint, head, cbw)
+    targets = north_star_ratios(variants, adjoint, head, cbw, planes)
 
     t_launch = head["us_per_step"] * 1e-6
     achieved = head["bytes_per_update"] * n * n / t_launch / 1e9
@@ -497,6 +544,7 @@ def main() -> int:
         "roofline": roof,
         "cpu_baseline": cpu,
         "copy_ceiling": ceil,
+        "plane_stream": planes,
         "pct_copy": head["pct_copy"],
         "targets": targets,
         "protocol": {"headline": f"{a.warm_time} s of load, {a.warmup} untimed steps, then "

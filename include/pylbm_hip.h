@@ -202,6 +202,18 @@ int pylbm_hip_copy(const void* src, void* dst, int64_t nbytes, void* stream);
 int pylbm_hip_stream_probe(int kind, int unroll, const void* src, void* dst, int64_t nbytes,
                            void* stream);
 
+/*
+ * Plane-stream probe: the access pattern of a two-buffer step without its arithmetic.
+ * src and dst each hold `planes` (9 or 19) planes of plane_bytes (a multiple of 256
+ * elements of elem_bytes = 2 or 4); lane x reads element x of every plane of src, then
+ * stores each to the same place in dst (non-temporal), with the step kernel's workgroup
+ * order.  dst becomes a copy of src.  Its rate is the ceiling of an S-plane SoA stream of
+ * that shape, beside the two-stream copy of pylbm_hip_copy.  Extends the copy probe of
+ * bench/roofline.py:30-51, which has one form only.
+ */
+int pylbm_hip_plane_probe(int planes, int elem_bytes, const void* src, void* dst,
+                          int64_t plane_bytes, void* stream);
+
 /* ---------------------------------------------------------------------------------------
  * Discrete adjoint of the two-buffer step (BGK / TRT, Guo forcing; params->aa must be 0).
  * No reference counterpart: the reference has no gradient path (its torch rungs run under

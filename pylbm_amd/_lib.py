@@ -33,6 +33,7 @@ EXPORTS = (
     "pylbm_hip_adjoint_workspace", "pylbm_hip_adjoint_seed", "pylbm_hip_adjoint_step",
     "pylbm_hip_adjoint_init", "pylbm_hip_run_store", "pylbm_hip_adjoint_run",
     "pylbm_hip_links", "pylbm_hip_reduce_noalloc", "pylbm_hip_stream_probe",
+    "pylbm_hip_plane_probe",
 )
 
 
@@ -103,6 +104,7 @@ def load() -> Any:
         "pylbm_hip_reduce_noalloc": ([pp, i32, vp, vp, vp, vp], i32),
         "pylbm_hip_copy": ([vp, vp, i64, vp], i32),
         "pylbm_hip_stream_probe": ([i32, i32, vp, vp, i64, vp], i32),
+        "pylbm_hip_plane_probe": ([i32, i32, vp, vp, i64, vp], i32),
         "pylbm_hip_adjoint_workspace": ([pp], i64),
         "pylbm_hip_adjoint_seed": ([pp, vp, vp, vp, vp, vp], i32),
         "pylbm_hip_adjoint_step": ([pp, vp, vp, vp, vp, vp, vp], i32),

@@ -45,6 +45,52 @@ def _event_times(fn, repeats: int) -> list[float]:
     return times
 
 
+def _load(fn, seconds: float) -> None:
+    """Run ``fn`` back to back for ``seconds`` of wall time: the clocks leave the idle
+    power state before anything is timed."""
+    torch = _torch()
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(8):
+            fn()
+        torch.cuda.synchronize()
+
+
+#: plane-stream shapes: name -> (planes, element bytes, elements per plane), the state
+#: arrays of the benchmark's workloads (D2Q9 4096^2 and D3Q19 512^3, fp32 and 16-bit)
+PLANE_SHAPES = {"d2q9_f32": (9, 4, 4096 * 4096), "d2q9_16": (9, 2, 4096 * 4096),
+                "d3q19_f32": (19, 4, 512 ** 3), "d3q19_16": (19, 2, 512 ** 3)}
+
+
+def plane_stream_bandwidth(shapes: dict | None = None, repeats: int = 20,
+                           load_s: float = 0.1) -> dict:
+    """Rate of the S-plane stream (``pylbm_hip_plane_probe``) at each shape: S planes read
+    and S written per element, counted as 2 S element bytes, the bytes a two-buffer step of
+    that lattice moves.  The ceiling for a step kernel of that shape: the probe has the
+    kernel's memory access pattern and no arithmetic."""
+    torch = _torch()
+    out: dict[str, Any] = {}
+    stream = int(torch.cuda.current_stream().cuda_stream)
+    for name, (planes, eb, n) in (shapes or PLANE_SHAPES).items():
+        plane = n * eb
+        a = torch.empty(planes * plane // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+
+        def fn(a=a, b=b, planes=planes, eb=eb, plane=plane):
+            _lib.call("pylbm_hip_plane_probe", planes, eb, a.data_ptr(), b.data_ptr(), plane,
+                      stream)
+
+        _load(fn, load_s)
+        times = _event_times(fn, repeats)
+        bw = [2 * planes * plane / t / 1e9 for t in times]
+        out[f"{name}_GBs"] = statistics.median(bw)
+        out[f"{name}_max_GBs"] = max(bw)
+        out[f"{name}_us"] = statistics.median(times) * 1e6
+        del a, b
+        torch.cuda.empty_cache()
+    return out
+
+
 def copy_bandwidth(mib: int = 4096, repeats: int = 20, sweep: bool = True) -> dict:
     """Median device copy bandwidth (read + write counted, bench/roofline.py:30-51).
 
@@ -74,6 +120,7 @@ def copy_bandwidth(mib: int = 4096, repeats: int = 20, sweep: bool = True) -> di
                                                    stream)),
                     2 if name.startswith("copy") else 1)
     out: dict[str, Any] = {"mib": mib}
+    _load(probes["hip_copy"][0], 0.25)
     for name, (fn, ways) in probes.items():
         times = _event_times(fn, repeats)
         bw = [ways * nbytes / t / 1e9 for t in times]

@@ -415,6 +415,24 @@ int pylbm_hip_stream_probe(int kind, int unroll, const void* src, void* dst, int
                   "pylbm_hip_stream_probe");
 }
 
+int pylbm_hip_plane_probe(int planes, int elem_bytes, const void* src, void* dst,
+                          int64_t plane_bytes, void* stream) {
+  if (planes != 9 && planes != 19)
+    return fail(PYLBM_EINVAL, "pylbm_hip_plane_probe: planes must be 9 or 19");
+  if (elem_bytes != 2 && elem_bytes != 4)
+    return fail(PYLBM_EINVAL, "pylbm_hip_plane_probe: elem_bytes must be 2 or 4");
+  if (!src || !dst) return fail(PYLBM_EINVAL, "pylbm_hip_plane_probe: NULL buffer");
+  if (plane_bytes < 0 || plane_bytes % ((int64_t)kBlock * elem_bytes))
+    return fail(PYLBM_EINVAL, "pylbm_hip_plane_probe: plane_bytes must be a non-negative "
+                "multiple of 256 elements");
+  if (plane_bytes / elem_bytes / kBlock >= ((int64_t)1 << 31))
+    return fail(PYLBM_ERANGE, "pylbm_hip_plane_probe: too many workgroups");
+  if (plane_bytes == 0) return 0;
+  return hip_fail(plbm::launch_plane_probe(planes, elem_bytes, src, dst, plane_bytes,
+                                           (hipStream_t)stream),
+                  "pylbm_hip_plane_probe");
+}
+
 int64_t pylbm_hip_adjoint_workspace(const pylbm_params* p) {
   if (check_params(p, "pylbm_hip_adjoint_workspace")) return -1;
   KArgs a;

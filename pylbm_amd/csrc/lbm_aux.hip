@@ -321,6 +321,42 @@ int probe_u(int kind, const void* src, void* dst, int64_t n16, hipStream_t s) {
   return (int)hipGetLastError();
 }
 
+// Plane-stream probe (pylbm_hip_plane_probe): the memory access pattern of a fused
+// two-buffer step without its arithmetic.  Lane x reads element x of each of S planes
+// (plane_elems apart, as the S populations of a node in the f[i, ...] layout), then writes
+// them to the same places in dst with non-temporal stores, in the step kernel's order (all
+// loads issued, then all stores).  Workgroups take the step kernel's XCD order: XCD b % 8
+// serves `chunk` consecutive row segments per turn.
+template <class T, int S>
+__global__ void __launch_bounds__(kBlock)
+k_plane_probe(const T* __restrict__ src, T* __restrict__ dst, int64_t plane_elems,
+              int chunk, FastDiv div_chunk) {
+  int b = (int)blockIdx.x;
+  if (chunk > 0) {
+    const int x = b & 7, i = b >> 3;
+    const int turn = (int)div_chunk.div((uint32_t)i);
+    b = (turn * 8 + x) * chunk + (i - turn * chunk);
+  }
+  const int64_t x = (int64_t)b * kBlock + threadIdx.x;  // plane_elems is a multiple of kBlock
+  T v[S];
+#pragma unroll
+  for (int k = 0; k < S; ++k) v[k] = src[k * plane_elems + x];
+#pragma unroll
+  for (int k = 0; k < S; ++k) __builtin_nontemporal_store(v[k], &dst[k * plane_elems + x]);
+}
+
+template <class T, int S>
+int plane_probe_t(const void* src, void* dst, int64_t plane_elems, hipStream_t s) {
+  const int64_t nblk = plane_elems / kBlock;
+  // the step kernel's order for large states: chunks of 16 (D3Q19), runs of 2 (D2Q9)
+  int chunk = S == 19 ? 16 : 2;
+  if (nblk % (8 * chunk) != 0) chunk = 0;
+  const FastDiv dc = make_fastdiv((uint32_t)(chunk > 0 ? chunk : 1));
+  k_plane_probe<T, S><<<dim3((unsigned)nblk), dim3(kBlock), 0, s>>>(
+      (const T*)src, (T*)dst, plane_elems, chunk, dc);
+  return (int)hipGetLastError();
+}
+
 inline unsigned nblocks(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 template <class L, int ST>
@@ -413,6 +449,19 @@ int reduce_scratch(double** out) {
   }
   *out = g_scratch_ptr[dev];
   return 0;
+}
+
+int launch_plane_probe(int planes, int elem_bytes, const void* src, void* dst,
+                       int64_t plane_bytes, hipStream_t s) {
+  const int64_t pe = plane_bytes / elem_bytes;
+  if (elem_bytes == 4) {
+    if (planes == 9) return plane_probe_t<float, 9>(src, dst, pe, s);
+    if (planes == 19) return plane_probe_t<float, 19>(src, dst, pe, s);
+  } else if (elem_bytes == 2) {
+    if (planes == 9) return plane_probe_t<uint16_t, 9>(src, dst, pe, s);
+    if (planes == 19) return plane_probe_t<uint16_t, 19>(src, dst, pe, s);
+  }
+  return PYLBM_EINVAL;
 }
 
 int launch_probe(int kind, int unroll, const void* src, void* dst, int64_t nbytes,

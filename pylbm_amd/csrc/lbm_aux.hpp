@@ -38,6 +38,8 @@ int reduce_scratch(double** out);
 int launch_copy(const void* src, void* dst, int64_t nbytes, hipStream_t s);
 // kind 0 copy (non-temporal), 1 copy (ordinary loads / stores), 2 read only, 3 write only;
 // unroll 1, 2, 4 or 8 (16-byte accesses per lane)
+int launch_plane_probe(int planes, int elem_bytes, const void* src, void* dst,
+                       int64_t plane_bytes, hipStream_t s);
 int launch_probe(int kind, int unroll, const void* src, void* dst, int64_t nbytes,
                  hipStream_t s);
 

@@ -199,7 +199,7 @@ def adjoint_leg(name: str, repeats: int = 3) -> dict:
     rng = np.random.default_rng(SEED)
     lam_storage = "f16" if name.endswith("-lam16") else "f32"
     if name == "channel":
-        nx, ny, T, C, storage, variant = 1024, 256, 2000, 100, "f32s", "fused"
+        nx, ny, T, C, storage, variant = 1024, 256, 2000, 100, "f32s", "fused-f32s"
         ff = np.zeros((2, ny, nx))
         ff[0] = 1e-6
         ff += 1e-7 * rng.standard_normal((2, ny, nx))

@@ -215,7 +215,11 @@ __device__ __forceinline__ RowLink row_link(const KArgs& a, int y, int z, int i,
 // gather = gather_issue (every load, back to back) + gather_finish (x-wall bounce-back in
 // one divergent block, decode); split so that a lane serving several nodes issues the
 // loads of all of them before it waits for any.
-template <class L, int ST, int PAT, int ADDR, int LA = 0>
+// LA: cache policy of the loads of links with no x component, LAX: of the x-shifted links
+// (a line an x-shifted link reads is also read by the neighbouring wave or row segment,
+// which may find it in the L2; the lines of a link with no x component belong to this
+// segment alone)
+template <class L, int ST, int PAT, int ADDR, int LA = 0, int LAX = LA>
 __device__ __forceinline__ void gather_issue(const KArgs& a, const Lane& l, const char* Ab,
                                              size_t plane, typename Codec<ST>::T (&t)[L::Q],
                                              float (&lid)[L::Q]) {
@@ -235,8 +239,12 @@ __device__ __forceinline__ void gather_issue(const KArgs& a, const Lane& l, cons
     }
     const RowLink r = row_link<L>(a, l.y, l.z, i, -1);
     const uint32_t offx = cx > 0 ? l.om : (cx < 0 ? l.op : l.o0);
-    t[i] = ld_pl<T, ADDR, LA>(Ab, plane, r.wall ? own_bb : far,
-                           r.wall ? 0 : r.delta * (ptrdiff_t)TS, r.wall ? l.o0 : offx);
+    if (cx == 0)
+      t[i] = ld_pl<T, ADDR, LA>(Ab, plane, r.wall ? own_bb : far,
+                             r.wall ? 0 : r.delta * (ptrdiff_t)TS, r.wall ? l.o0 : offx);
+    else
+      t[i] = ld_pl<T, ADDR, LAX>(Ab, plane, r.wall ? own_bb : far,
+                              r.wall ? 0 : r.delta * (ptrdiff_t)TS, r.wall ? l.o0 : offx);
     if (cx != 0) lid[i] = (r.wall && r.top) ? (float)(6.0 * L::w(i) * cx) * a.ulid : 0.0f;
   }
 }

@@ -38,13 +38,18 @@ def test_ladder_takes_each_rungs_largest_size():
 
 def test_headline_numbers_come_from_the_record():
     d2 = [r for r in report.ladder(BENCH, "D2Q9") if r["backend"] != "numpy"]
-    best32 = max((r for r in d2 if "16" not in r["variant"]), key=lambda r: r["median"])
-    best16 = max((r for r in d2 if "16" in r["variant"]), key=lambda r: r["median"])
+    pull = [r for r in d2 if "-aa" not in r["variant"]]
+    aa = [r for r in d2 if "-aa" in r["variant"]]
+    best32 = max((r for r in pull if "16" not in r["variant"]), key=lambda r: r["median"])
+    best16 = max((r for r in pull if "16" in r["variant"]), key=lambda r: r["median"])
+    aa32 = max((r for r in aa if "16" not in r["variant"]), key=lambda r: r["median"])
     lines = report.headline(BENCH)
     assert f"{best32['median']:,.0f} MLUPS" in lines[0]
     assert f"{best32['backend']}/{best32['variant']}" in lines[0]
     assert f"{best16['median']:,.0f} MLUPS" in lines[1]
     assert f"{best16['median'] / best32['median']:.2f}x" in lines[1]
+    # the AA kernels are reported on their own line, marked as cache-assisted
+    assert "cache-assisted" in lines[2] and f"{aa32['median']:,.0f} MLUPS" in lines[2]
     # the float64 NumPy rung is never the headline
     assert "numpy" not in " ".join(lines)
 
@@ -105,6 +110,13 @@ def test_bench_line_converts_to_the_schema(tmp_path):
     assert [report.rung_key(r) for r in report.ladder(rec, "D2Q9", "trt")] == []
     assert len(report.ladder(rec, "D2Q9", "trt", workload=True)) == 1
     assert "80,000 MLUPS" in report.render(rec)
+    # the metric's own number leads the headline; its duplicate leg stays out of the table
+    line["variants"].append(dict(line["variants"][0], MLUPS=79000.0, GBs=5688.0,
+                                 protocol="headline"))
+    line["config"] = {"workload": "D2Q9 4096^2"}
+    rec = report.from_bench_line(line)
+    assert len(rec["runs"]) == 2 and rec["headline"]["value"] == 80000.0
+    assert report.headline(rec)[0].startswith("bench.py on MI355X: **80,000 MLUPS**")
 
 
 def test_cli_report_writes_markdown(tmp_path):

@@ -342,6 +342,9 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true", help="skip the forward+adjoint legs")
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only the forward+adjoint legs (prints them, no JSON line)")
+    ap.add_argument("--cache-sweep", action="store_true",
+                    help="only the stream bandwidth against buffer size, 16 MiB - 4 GiB "
+                         "(prints a table, no JSON line)")
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--min-time", type=float, default=0.1,
                     help="every non-headline leg: seconds per timed window (default 0.1)")
@@ -369,6 +372,14 @@ def main() -> int:
     _lib.load()
     dev = device_name()
     log(f"device {dev}")
+    if a.cache_sweep:
+        from pylbm_amd.benchmark import cache_sweep
+
+        log("stream bandwidth against buffer size (GB/s; copy = read + write)")
+        for r in cache_sweep():
+            log(f"{r['mib']:5d} MiB  copy nt {r['copy_nt_GBs']:6.0f}  copy {r['copy_GBs']:6.0f}  "
+                f"read {r['read_GBs']:6.0f}")
+        return 0
     if a.adjoint_only:
         for case in ADJOINT_CASES:
             r = adjoint_leg(case)

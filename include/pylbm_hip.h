@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define PYLBM_HIP_ABI_VERSION 3
+#define PYLBM_HIP_ABI_VERSION 4
 
 enum { PYLBM_D2Q9 = 0, PYLBM_D3Q19 = 1 };
 enum { PYLBM_BGK = 0, PYLBM_TRT = 1, PYLBM_MRT = 2 };
@@ -114,6 +114,20 @@ int pylbm_hip_run(const pylbm_params* p, void* a, void* b, int64_t nsteps, void*
  */
 int pylbm_hip_run_store(const pylbm_params* p, const void* src, void* const* dst,
                         int64_t nsteps, void* stream);
+
+/*
+ * pylbm_hip_run_store that also writes moment records: when rec[k] (a host array of nsteps
+ * device pointers, or rec itself) is not NULL, step k writes to rec[k] the density and
+ * velocity its collision used at every node, (rho, u_x, u_y, u_z) fp32 (u = (sum c g + F/2)
+ * / rho of the pulled populations g; u_z = 0 for D2Q9), node-major: N x 16 bytes.  The
+ * adjoint of a rollout that does not differentiate tau needs only these
+ * (pylbm_hip_adjoint_run_record), 16 bytes per node instead of the stored state.
+ * BGK / TRT box problems whose (q, N) fp32 array spans less than 4 GiB (PYLBM_ERANGE
+ * otherwise).  No reference counterpart (the reference has no rollout storage); a
+ * generalisation of FusedSolver._advance (pylbm/backends/_fused.py:116-124).
+ */
+int pylbm_hip_run_record(const pylbm_params* p, const void* src, void* const* dst,
+                         void* const* rec, int64_t nsteps, void* stream);
 
 /*
  * Canonical float64 state (q, N) -> stored layout (shift by -w_i for 16-bit storage,
@@ -252,6 +266,16 @@ int pylbm_hip_adjoint_step(const pylbm_params* p, const void* f_t, const void* l
 int pylbm_hip_adjoint_run(const pylbm_params* p, const void* const* states, int64_t nsteps,
                           void* lam_a, void* lam_b, float* dF, double* dtau_part, int* which,
                           void* stream);
+
+/* pylbm_hip_adjoint_step and pylbm_hip_adjoint_run with the forward quantities taken from
+ * the steps' moment records (pylbm_hip_run_record) instead of the stored states: the same
+ * adjoint algebra, one 16-byte load per node in place of the q-population pull.  No rate
+ * sensitivities (they need the pulled populations): for rollouts that do not differentiate
+ * tau.  Sizes as pylbm_hip_run_record. */
+int pylbm_hip_adjoint_step_record(const pylbm_params* p, const void* rec_t, const void* lam_in,
+                                  void* lam_out, float* dF, void* stream);
+int pylbm_hip_adjoint_run_record(const pylbm_params* p, const void* const* recs, int64_t nsteps,
+                                 void* lam_a, void* lam_b, float* dF, int* which, void* stream);
 
 /* Gradient of the initial equilibrium f0 = feq(rho0, u0) (float64 fields; NULL rho0 = 1,
  * NULL u0 = 0): du0 (d, N) and drho0 (N) in float64 from lam0 = dL/df0 (either output may

@@ -12,7 +12,7 @@ from pathlib import Path
 from typing import Any
 
 LIB_PATH = Path(__file__).resolve().parent / "libpylbm_hip.so"
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 D2Q9, D3Q19 = 0, 1
 BGK, TRT, MRT = 0, 1, 2
@@ -33,7 +33,8 @@ EXPORTS = (
     "pylbm_hip_adjoint_workspace", "pylbm_hip_adjoint_seed", "pylbm_hip_adjoint_step",
     "pylbm_hip_adjoint_init", "pylbm_hip_run_store", "pylbm_hip_adjoint_run",
     "pylbm_hip_links", "pylbm_hip_reduce_noalloc", "pylbm_hip_stream_probe",
-    "pylbm_hip_plane_probe",
+    "pylbm_hip_plane_probe", "pylbm_hip_run_record", "pylbm_hip_adjoint_step_record",
+    "pylbm_hip_adjoint_run_record",
 )
 
 
@@ -113,6 +114,11 @@ def load() -> Any:
         "pylbm_hip_adjoint_run": ([pp, ctypes.POINTER(vp), i64, vp, vp, vp, vp,
                                    ctypes.POINTER(ctypes.c_int), vp], i32),
         "pylbm_hip_links": ([pp, vp, vp, vp], i32),
+        "pylbm_hip_run_record": ([pp, vp, ctypes.POINTER(vp), ctypes.POINTER(vp), i64, vp],
+                                 i32),
+        "pylbm_hip_adjoint_step_record": ([pp, vp, vp, vp, vp, vp], i32),
+        "pylbm_hip_adjoint_run_record": ([pp, ctypes.POINTER(vp), i64, vp, vp, vp,
+                                          ctypes.POINTER(ctypes.c_int), vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -1,6 +1,6 @@
 """Differentiable K-step rollouts on the GPU: checkpointed forward + fused HIP adjoint.
 
-:class:`Rollout` advances a box problem ``steps`` steps with the two-buffer fused kernels
+:func:`rollout` advances a box problem ``steps`` steps with the two-buffer fused kernels
 (fp32, fp16 or bf16 storage) and returns the final post-collision velocity
 u = (sum c f - F/2) / rho.  :func:`rollout` wraps it in ``torch.autograd.Function`` so any
 torch loss of u back-propagates to
@@ -10,14 +10,24 @@ torch loss of u back-propagates to
 * the initial velocity (and density) of the equilibrium start state.
 
 Forward states are checkpointed in HBM every ``checkpoint`` steps; the backward pass
-recomputes one segment at a time (all its states kept in HBM), then runs the adjoint steps
-of that segment in reverse with ``pylbm_hip_adjoint_step``.  The adjoint states are fp32 by
-default; ``lam_storage="f16"`` stores them as fp16 (all adjoint arithmetic stays fp32), which
-cuts an adjoint step's traffic from q*(s + 8) to q*(s + 4) bytes per node at a gradient error
-of a few 1e-4 relative (measured against float64 in the tests).  The last segment's states are
-kept from the forward pass, so it is not recomputed.  The reference has no gradient path;
-the gradients are checked against float64 torch autograd of a restated step and against
-central finite differences.
+recomputes one segment at a time, then runs the adjoint steps of that segment in reverse
+(``pylbm_hip_adjoint_run``).  What the recompute keeps for the adjoint depends on the
+gradients asked for:
+
+* tau not differentiated (the default ``records=None``): every step's moment record, the
+  density and velocity its collision used (16 bytes per node,
+  ``pylbm_hip_run_record``), while the state itself ping-pongs between two buffers; the
+  adjoint reads the records (``pylbm_hip_adjoint_run_record``);
+* tau differentiated: every state of the segment (2 q or 4 q bytes per node), since the
+  rate sensitivities need the pulled populations.
+
+The adjoint states are fp32 by default; ``lam_storage="f16"`` stores them as fp16 (all
+adjoint arithmetic stays fp32), which cuts an adjoint step's traffic from q*(s + 8) to
+q*(s + 4) bytes per node at a gradient error of a few 1e-4 relative (measured against
+float64 in the tests).  The last segment's states (or records) are kept from the forward
+pass, so it is not recomputed.  The reference has no gradient path; the gradients are
+checked against float64 torch autograd of a restated step and against central finite
+differences.
 """
 
 from __future__ import annotations
@@ -81,8 +91,22 @@ class RolloutConfig:
     #: element type of the adjoint states between steps: "f32" or "f16" (fp32 arithmetic)
     lam_storage: str = "f32"
     #: recompute the next checkpoint segment on a second stream during each segment's
-    #: adjoint (False: one stream, one after the other; the gradients are the same)
-    overlap: bool = True
+    #: adjoint (False: one stream, one after the other; the gradients are the same).
+    #: None (default): overlap unless the adjoint reads moment records on a grid of 2^20
+    #: nodes or more.  On small grids neither kernel fills the GPU alone (1024 x 256
+    #: channel: 25.4% -> 28.8% of the forward rate); on large ones both are bandwidth bound
+    #: and run concurrently they evict each other's working set from the Infinity Cache
+    #: (128^3 box with records: 60.0 ms side by side, 55.6 ms one after the
+    #: other; with stored states 59.0 against 60.3 ms)
+    overlap: bool | None = None
+    #: what the adjoint reads of each forward step: None (default) moment records when tau
+    #: is not differentiated, else the stored states; False always the stored states; True
+    #: always records (then tau must not be differentiated).  A record is the density and
+    #: velocity the step's collision used, 16 bytes per node, which is all the adjoint
+    #: needs unless it also sums the rate sensitivities; the stored state is 2 q bytes
+    #: (16-bit) or 4 q bytes per node, and the recompute of a segment then keeps its state
+    #: in two buffers instead of one per step.
+    records: bool | None = None
 
     def __post_init__(self) -> None:
         p = self.problem
@@ -136,6 +160,10 @@ class _Engine:
     def state(self) -> torch.Tensor:
         return torch.empty(self.q * self.N, device=self.dev, dtype=self.dtype)
 
+    def record(self) -> torch.Tensor:
+        """Moment records of one step: (rho, u_x, u_y, u_z) fp32 per node."""
+        return torch.empty(4 * self.N, device=self.dev, dtype=torch.float32)
+
     def init(self, rho0: torch.Tensor | None, u0: torch.Tensor | None, f: torch.Tensor) -> None:
         _lib.call("pylbm_hip_init_equilibrium", self.ref, _p(rho0), _p(u0), _p(f), _s())
 
@@ -149,9 +177,10 @@ class _Engine:
 
 
 def _ptr_array(tensors: list) -> Any:
+    """Host array of device pointers (None -> NULL)."""
     arr = (ctypes.c_void_p * max(len(tensors), 1))()
     for k, t in enumerate(tensors):
-        arr[k] = t.data_ptr()
+        arr[k] = None if t is None else t.data_ptr()
     return arr
 
 
@@ -166,30 +195,47 @@ class _RolloutFn(torch.autograd.Function):
         T, C = cfg.steps, cfg.checkpoint
         K = (T + C - 1) // C
         last0 = (K - 1) * C  # first step of the last segment
+        need_tau = bool(ctx.needs_input_grad[1])
+        records = (not need_tau) if cfg.records is None else bool(cfg.records)
+        if records and need_tau:
+            raise ValueError("records=True: the tau gradient needs the stored states")
         f0 = eng.state()
         eng.init(r0d, u0d, f0)
-        # where each f^{t+1} goes: its own buffer if it is a checkpoint, an input of the last
-        # segment or f^T, else one of two scratch buffers (never the buffer it is read from)
+        # where each f^{t+1} goes: its own buffer if it is a checkpoint or f^T (or, without
+        # records, an input of the last segment), else one of two scratch buffers (never
+        # the buffer it is read from); with records, the steps of the last segment write
+        # their records
         scratch = [eng.state(), eng.state()]
-        ckpts, last, dst = [f0], [], []
+        ckpts, last, dst, rec = [f0], [], [], []
         cur = f0
         for t in range(T):
             if t >= last0:
-                last.append(cur)
-            keep = (t + 1 == T) or ((t + 1) % C == 0) or (t + 1 >= last0)
+                if records:
+                    rec.append(eng.record())
+                    last.append(rec[-1])
+                else:
+                    last.append(cur)
+            else:
+                rec.append(None)
+            keep = (t + 1 == T) or ((t + 1) % C == 0) or (not records and t + 1 >= last0)
             nxt = eng.state() if keep else (scratch[0] if cur is not scratch[0] else scratch[1])
             if t + 1 < T and (t + 1) % C == 0:
                 ckpts.append(nxt)
             dst.append(nxt)
             cur = nxt
-        _lib.call("pylbm_hip_run_store", eng.ref, _p(f0), _ptr_array(dst), T, _s())
-        del scratch, dst
+        if records:
+            _lib.call("pylbm_hip_run_record", eng.ref, _p(f0), _ptr_array(dst),
+                      _ptr_array(rec), T, _s())
+        else:
+            _lib.call("pylbm_hip_run_store", eng.ref, _p(f0), _ptr_array(dst), T, _s())
+        del scratch, dst, rec
         fT = cur
         u = eng.velocity(fT)
         ctx.eng, ctx.ckpts, ctx.last, ctx.fT = eng, ckpts, last, fT
+        ctx.records = records
         ctx.u0d, ctx.r0d = u0d, r0d
-        ctx.needs = (force_field is not None and ctx.needs_input_grad[0],
-                     ctx.needs_input_grad[1], u0 is not None and ctx.needs_input_grad[2],
+        ctx.needs = (force_field is not None and ctx.needs_input_grad[0], need_tau,
+                     u0 is not None and ctx.needs_input_grad[2],
                      rho0 is not None and ctx.needs_input_grad[3])
         return u
 
@@ -225,23 +271,37 @@ class _RolloutFn(torch.autograd.Function):
         # are independent, and on small grids neither fills the GPU alone.  Every adjoint
         # step (and so every dF / partial-sum accumulation) stays on the main stream in
         # reverse order, so the gradients do not depend on the overlap.
+        records = ctx.records
+        overlap = cfg.overlap
+        if overlap is None:
+            overlap = not records or eng.N < (1 << 20)
         main = torch.cuda.current_stream(eng.dev)
-        side = torch.cuda.Stream(device=eng.dev) if cfg.overlap else main
+        side = torch.cuda.Stream(device=eng.dev) if overlap else main
         side.wait_stream(main)  # the checkpoints and the seed
+        # per buffer set: the segment's stored states (n - 1 of them: the checkpoint is the
+        # first), or, with records, two scratch states and the n steps' records
         seg_sets: list[list[torch.Tensor]] = [[], []]
+        seg_recs: list[list[torch.Tensor]] = [[], []]
         ready: dict[int, torch.cuda.Event] = {}
         done: dict[int, torch.cuda.Event] = {}
 
         def recompute(k: int) -> None:
             n = min(C, T - k * C)
-            buf = seg_sets[k % 2]
-            while len(buf) < n - 1:
+            buf, recs = seg_sets[k % 2], seg_recs[k % 2]
+            while len(buf) < (2 if records else n - 1):
                 buf.append(eng.state())
+            while records and len(recs) < n:
+                recs.append(eng.record())
             if k + 2 in done:  # the adjoint of segment k + 2 read this buffer set
                 side.wait_event(done[k + 2])
             with torch.cuda.stream(side):
-                _lib.call("pylbm_hip_run_store", eng.ref, _p(ctx.ckpts[k]),
-                          _ptr_array(buf[:n - 1]), n - 1, int(side.cuda_stream))
+                if records:
+                    _lib.call("pylbm_hip_run_record", eng.ref, _p(ctx.ckpts[k]),
+                              _ptr_array([buf[j % 2] for j in range(n)]), _ptr_array(recs[:n]),
+                              n, int(side.cuda_stream))
+                else:
+                    _lib.call("pylbm_hip_run_store", eng.ref, _p(ctx.ckpts[k]),
+                              _ptr_array(buf[:n - 1]), n - 1, int(side.cuda_stream))
             ev = torch.cuda.Event()
             ev.record(side)
             ready[k] = ev
@@ -249,15 +309,21 @@ class _RolloutFn(torch.autograd.Function):
         for k in range(K - 1, -1, -1):
             n = min(C, T - k * C)
             if k == K - 1:
-                states = ctx.last
+                inputs = ctx.last
             else:
                 main.wait_event(ready.pop(k))
-                states = [ctx.ckpts[k]] + seg_sets[k % 2][:n - 1]
+                inputs = seg_recs[k % 2][:n] if records else \
+                    [ctx.ckpts[k]] + seg_sets[k % 2][:n - 1]
             if k >= 1:  # enqueued before this segment's adjoint, so the two overlap
                 recompute(k - 1)
-            _lib.call("pylbm_hip_adjoint_run", eng.ref, _ptr_array(states), n, _p(lam[cur]),
-                      _p(lam[1 - cur]), _p(dF), _p(part), ctypes.byref(which),
-                      int(main.cuda_stream))
+            if records:
+                _lib.call("pylbm_hip_adjoint_run_record", eng.ref, _ptr_array(inputs), n,
+                          _p(lam[cur]), _p(lam[1 - cur]), _p(dF), ctypes.byref(which),
+                          int(main.cuda_stream))
+            else:
+                _lib.call("pylbm_hip_adjoint_run", eng.ref, _ptr_array(inputs), n,
+                          _p(lam[cur]), _p(lam[1 - cur]), _p(dF), _p(part),
+                          ctypes.byref(which), int(main.cuda_stream))
             cur = cur if which.value == 0 else 1 - cur
             ev = torch.cuda.Event()
             ev.record(main)

@@ -139,6 +139,37 @@ def copy_bandwidth(mib: int = 4096, repeats: int = 20, sweep: bool = True) -> di
     return out
 
 
+def cache_sweep(sizes_mib: tuple = (16, 32, 64, 96, 128, 192, 256, 512, 4096),
+                repeats: int = 20) -> list[dict]:
+    """Stream bandwidth against buffer size: the copy (non-temporal and ordinary accesses)
+    and the read-only stream of ``pylbm_hip_stream_probe`` at one 16-byte access per lane,
+    over buffers from well inside the 256 MB Infinity Cache to far beyond it (median of
+    ``repeats`` launches after a 0.05 s load phase per probe; a copy counts read + write)."""
+    torch = _torch()
+    stream = int(torch.cuda.current_stream().cuda_stream)
+    rows = []
+    for mib in sizes_mib:
+        nbytes = mib << 20
+        a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        # read-only stream: at most one 16-byte word per workgroup lands in dst
+        r = torch.empty(nbytes // 1024 + 64, dtype=torch.float32, device="cuda")
+        row: dict[str, Any] = {"mib": mib}
+        for name, kind, dst, ways in (("copy_nt", 0, b, 2), ("copy", 1, b, 2),
+                                      ("read", 2, r, 1)):
+            def fn(kind=kind, dst=dst):
+                _lib.call("pylbm_hip_stream_probe", kind, 1, a.data_ptr(), dst.data_ptr(),
+                          nbytes, stream)
+            _load(fn, 0.05)
+            t = statistics.median(_event_times(fn, repeats))
+            row[f"{name}_GBs"] = ways * nbytes / t / 1e9
+            row[f"{name}_us"] = t * 1e6
+        rows.append(row)
+        del a, b, r
+        torch.cuda.empty_cache()
+    return rows
+
+
 _SMI: dict[str, Any] = {}
 
 

@@ -162,6 +162,7 @@ int make_kargs(const pylbm_params* p, KArgs& a, const char* what) {
   a.links = p->links;
   a.inlet = p->inlet;
   a.outlet = p->outlet;
+  a.rec = nullptr;
   return 0;
 }
 
@@ -179,6 +180,49 @@ int step_impl(const pylbm_params* p, const KArgs& a, const void* src, void* dst,
   if (p->lattice == PYLBM_D2Q9)
     return launch_step<D2Q9>(a, p->collision, p->storage, pattern, fmode, src, dst, s);
   return launch_step<D3Q19>(a, p->collision, p->storage, pattern, fmode, src, dst, s);
+}
+
+// The record mode of the forward steps and the adjoint: the record array is N x 16 bytes and
+// the kernels address it, the adjoint states and every other array through one descriptor
+// each, so a q-plane fp32 array must span less than 4 GiB.
+int check_records(const pylbm_params* p, const char* what) {
+  const int64_t q = p->lattice == PYLBM_D2Q9 ? 9 : 19;
+  if (q * p->n_nodes * 4 >= ((int64_t)1 << 32))
+    return fail(PYLBM_ERANGE, "%s: moment records need q * N * 4 bytes under 4 GiB", what);
+  if (p->collision == PYLBM_MRT)
+    return fail(PYLBM_ENOTSUP, "%s: moment records cover BGK and TRT", what);
+  if (has_geometry(p))
+    return fail(PYLBM_ENOTSUP, "%s: moment records cover box geometries only", what);
+  return 0;
+}
+
+// The stepping loop of pylbm_hip_run_store / pylbm_hip_run_record: step k reads the
+// previous state (src for k = 0) and writes dst[k]; with rec, a non-NULL rec[k] also
+// receives step k's moment records.
+int run_store_impl(const pylbm_params* p, const void* src, void* const* dst, void* const* rec,
+                   int64_t nsteps, void* stream, const char* what) {
+  if (int e = check_params(p, what)) return e;
+  if (p->aa) return fail(PYLBM_EINVAL, "%s: two-buffer layout only", what);
+  if (nsteps < 0) return fail(PYLBM_EINVAL, "%s: nsteps < 0", what);
+  if (nsteps > 0 && (!src || !dst)) return fail(PYLBM_EINVAL, "%s: NULL buffer", what);
+  if (rec)
+    if (int e = check_records(p, what)) return e;
+  KArgs a;
+  if (int e = make_kargs(p, a, what)) return e;
+  KArgs ar = a;
+  ar.reverse = 1;
+  const void* cur = src;
+  for (int64_t k = 0; k < nsteps; ++k) {
+    if (!dst[k] || dst[k] == cur)
+      return fail(PYLBM_EINVAL, "%s: dst[%lld] is NULL or equals its source", what,
+                  (long long)k);
+    KArgs ak = ((k & 1) && a.resident) ? ar : a;
+    ak.rec = rec ? static_cast<float*>(rec[k]) : nullptr;
+    if (int e = step_impl(p, ak, cur, dst[k], PYLBM_AB, (hipStream_t)stream))
+      return hip_fail(e, what);
+    cur = dst[k];
+  }
+  return 0;
 }
 
 AuxArgs aux_from(const pylbm_params* p) {
@@ -261,25 +305,12 @@ int pylbm_hip_run(const pylbm_params* p, void* a_buf, void* b_buf, int64_t nstep
 
 int pylbm_hip_run_store(const pylbm_params* p, const void* src, void* const* dst,
                         int64_t nsteps, void* stream) {
-  if (int e = check_params(p, "pylbm_hip_run_store")) return e;
-  if (p->aa) return fail(PYLBM_EINVAL, "pylbm_hip_run_store: two-buffer layout only");
-  if (nsteps < 0) return fail(PYLBM_EINVAL, "pylbm_hip_run_store: nsteps < 0");
-  if (nsteps > 0 && (!src || !dst)) return fail(PYLBM_EINVAL, "pylbm_hip_run_store: NULL buffer");
-  KArgs a;
-  if (int e = make_kargs(p, a, "pylbm_hip_run_store")) return e;
-  KArgs ar = a;
-  ar.reverse = 1;
-  const void* cur = src;
-  for (int64_t k = 0; k < nsteps; ++k) {
-    if (!dst[k] || dst[k] == cur)
-      return fail(PYLBM_EINVAL, "pylbm_hip_run_store: dst[%lld] is NULL or equals its source",
-                  (long long)k);
-    if (int e = step_impl(p, ((k & 1) && a.resident) ? ar : a, cur, dst[k], PYLBM_AB,
-                          (hipStream_t)stream))
-      return hip_fail(e, "pylbm_hip_run_store");
-    cur = dst[k];
-  }
-  return 0;
+  return run_store_impl(p, src, dst, nullptr, nsteps, stream, "pylbm_hip_run_store");
+}
+
+int pylbm_hip_run_record(const pylbm_params* p, const void* src, void* const* dst,
+                         void* const* rec, int64_t nsteps, void* stream) {
+  return run_store_impl(p, src, dst, rec, nsteps, stream, "pylbm_hip_run_record");
 }
 
 int pylbm_hip_encode(const pylbm_params* p, const double* canon, void* f, void* stream) {
@@ -454,6 +485,64 @@ static int check_adjoint(const pylbm_params* p, const char* what) {
   return 0;
 }
 
+static int adjoint_step_impl(const pylbm_params* p, const void* f_t, bool rec,
+                             const void* lam_in, void* lam_out, float* dF, double* dtau_part,
+                             void* stream, const char* what) {
+  if (int e = check_adjoint(p, what)) return e;
+  if (rec)
+    if (int e = check_records(p, what)) return e;
+  if (!f_t || !lam_in || !lam_out) return fail(PYLBM_EINVAL, "%s: NULL buffer", what);
+  if (lam_in == lam_out) return fail(PYLBM_EINVAL, "%s: lam_in and lam_out must differ", what);
+  if (p->force_field && !dF) return fail(PYLBM_EINVAL, "%s: a force field needs dF", what);
+  KArgs a;
+  if (int e = make_kargs(p, a, what)) return e;
+  return hip_fail(launch_adjoint_step(p->lattice, a, p->collision, p->storage, p->lam_storage,
+                                      PYLBM_AB, p->force_field != nullptr, rec, f_t, lam_in,
+                                      lam_out, dF, dtau_part, (hipStream_t)stream),
+                  what);
+}
+
+static int adjoint_run_impl(const pylbm_params* p, const void* const* inputs, bool rec,
+                            int64_t nsteps, void* lam_a, void* lam_b, float* dF,
+                            double* dtau_part, int* which, void* stream, const char* what) {
+  if (int e = check_adjoint(p, what)) return e;
+  if (rec)
+    if (int e = check_records(p, what)) return e;
+  if (nsteps < 0 || !which) return fail(PYLBM_EINVAL, "%s: bad arguments", what);
+  if (nsteps > 0 && (!inputs || !lam_a || !lam_b || lam_a == lam_b))
+    return fail(PYLBM_EINVAL, "%s: NULL or aliased buffer", what);
+  if (p->force_field && !dF) return fail(PYLBM_EINVAL, "%s: a force field needs dF", what);
+  KArgs a;
+  if (int e = make_kargs(p, a, what)) return e;
+  KArgs ar = a;
+  ar.reverse = 1;
+  for (int64_t k = 0; k < nsteps; ++k)
+    if (!inputs[k]) return fail(PYLBM_EINVAL, "%s: input %lld is NULL", what, (long long)k);
+  // The steps run in pairs in place in lam_a (the AA pattern of the adjoint, k_adjoint):
+  // one adjoint-state array instead of two, which a state resident in the Infinity Cache
+  // keeps there.  An odd last step goes lam_a -> lam_b.
+  void* lin = lam_a;
+  void* lout = lam_b;
+  for (int64_t k = nsteps - 1; k >= 0; --k) {
+    // the sweep direction alternates every step for resident states (remap_block)
+    const bool rev = ((nsteps - 1 - k) & 1) && a.resident;
+    const bool pair = (nsteps - 1 - k) < nsteps - nsteps % 2;
+    const int pat = !pair ? PYLBM_AB : (((nsteps - 1 - k) & 1) ? PYLBM_AA_ODD : PYLBM_AA_EVEN);
+    if (int e = launch_adjoint_step(p->lattice, rev ? ar : a, p->collision, p->storage,
+                                    p->lam_storage, pat, p->force_field != nullptr, rec,
+                                    inputs[k], lin, pair ? lin : lout, dF, dtau_part,
+                                    (hipStream_t)stream))
+      return hip_fail(e, what);
+    if (!pair) {
+      void* t = lin;
+      lin = lout;
+      lout = t;
+    }
+  }
+  *which = (lin == lam_a) ? 0 : 1;
+  return 0;
+}
+
 int pylbm_hip_adjoint_seed(const pylbm_params* p, const void* f_T, const float* grad_u,
                            void* lam, float* dF, void* stream) {
   if (int e = check_adjoint(p, "pylbm_hip_adjoint_seed")) return e;
@@ -470,59 +559,27 @@ int pylbm_hip_adjoint_seed(const pylbm_params* p, const void* f_T, const float* 
 
 int pylbm_hip_adjoint_step(const pylbm_params* p, const void* f_t, const void* lam_in,
                            void* lam_out, float* dF, double* dtau_part, void* stream) {
-  if (int e = check_adjoint(p, "pylbm_hip_adjoint_step")) return e;
-  if (!f_t || !lam_in || !lam_out)
-    return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_step: NULL buffer");
-  if (lam_in == lam_out)
-    return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_step: lam_in and lam_out must differ");
-  if (p->force_field && !dF)
-    return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_step: a force field needs dF");
-  KArgs a;
-  if (int e = make_kargs(p, a, "pylbm_hip_adjoint_step")) return e;
-  return hip_fail(launch_adjoint_step(p->lattice, a, p->collision, p->storage, p->lam_storage,
-                                      PYLBM_AB, p->force_field != nullptr, f_t, lam_in, lam_out,
-                                      dF, dtau_part, (hipStream_t)stream),
-                  "pylbm_hip_adjoint_step");
+  return adjoint_step_impl(p, f_t, false, lam_in, lam_out, dF, dtau_part, stream,
+                           "pylbm_hip_adjoint_step");
+}
+
+int pylbm_hip_adjoint_step_record(const pylbm_params* p, const void* rec_t, const void* lam_in,
+                                  void* lam_out, float* dF, void* stream) {
+  return adjoint_step_impl(p, rec_t, true, lam_in, lam_out, dF, nullptr, stream,
+                           "pylbm_hip_adjoint_step_record");
 }
 
 int pylbm_hip_adjoint_run(const pylbm_params* p, const void* const* states, int64_t nsteps,
                           void* lam_a, void* lam_b, float* dF, double* dtau_part, int* which,
                           void* stream) {
-  if (int e = check_adjoint(p, "pylbm_hip_adjoint_run")) return e;
-  if (nsteps < 0 || !which) return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_run: bad arguments");
-  if (nsteps > 0 && (!states || !lam_a || !lam_b || lam_a == lam_b))
-    return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_run: NULL or aliased buffer");
-  if (p->force_field && !dF)
-    return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_run: a force field needs dF");
-  KArgs a;
-  if (int e = make_kargs(p, a, "pylbm_hip_adjoint_run")) return e;
-  KArgs ar = a;
-  ar.reverse = 1;
-  for (int64_t k = 0; k < nsteps; ++k)
-    if (!states[k]) return fail(PYLBM_EINVAL, "pylbm_hip_adjoint_run: states[%lld] is NULL",
-                                (long long)k);
-  // The steps run in pairs in place in lam_a (the AA pattern of the adjoint, k_adjoint):
-  // one adjoint-state array instead of two, which a state resident in the Infinity Cache
-  // keeps there.  An odd last step goes lam_a -> lam_b.
-  void* lin = lam_a;
-  void* lout = lam_b;
-  for (int64_t k = nsteps - 1; k >= 0; --k) {
-    // the sweep direction alternates every step for resident states (remap_block)
-    const bool rev = ((nsteps - 1 - k) & 1) && a.resident;
-    const bool pair = (nsteps - 1 - k) < nsteps - nsteps % 2;
-    const int pat = !pair ? PYLBM_AB : (((nsteps - 1 - k) & 1) ? PYLBM_AA_ODD : PYLBM_AA_EVEN);
-    if (int e = launch_adjoint_step(p->lattice, rev ? ar : a, p->collision, p->storage,
-                                    p->lam_storage, pat, p->force_field != nullptr, states[k],
-                                    lin, pair ? lin : lout, dF, dtau_part, (hipStream_t)stream))
-      return hip_fail(e, "pylbm_hip_adjoint_run");
-    if (!pair) {
-      void* t = lin;
-      lin = lout;
-      lout = t;
-    }
-  }
-  *which = (lin == lam_a) ? 0 : 1;
-  return 0;
+  return adjoint_run_impl(p, states, false, nsteps, lam_a, lam_b, dF, dtau_part, which, stream,
+                          "pylbm_hip_adjoint_run");
+}
+
+int pylbm_hip_adjoint_run_record(const pylbm_params* p, const void* const* recs, int64_t nsteps,
+                                 void* lam_a, void* lam_b, float* dF, int* which, void* stream) {
+  return adjoint_run_impl(p, recs, true, nsteps, lam_a, lam_b, dF, nullptr, which, stream,
+                          "pylbm_hip_adjoint_run_record");
 }
 
 int pylbm_hip_adjoint_init(const pylbm_params* p, const double* rho0, const double* u0,

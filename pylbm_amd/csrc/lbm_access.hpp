@@ -48,6 +48,33 @@ __device__ __forceinline__ void st_at(char* base, uint32_t off, T v) {
                                           AUX);
 }
 
+// Moment record of one node: (rho, u_x, u_y, u_z) fp32, 16 bytes at byte offset off of
+// the node-major record array (one 16-byte access per lane, so a wave moves 1 KiB).
+typedef unsigned int rec4_t __attribute__((ext_vector_type(4)));
+
+template <int AUX>
+__device__ __forceinline__ void st_rec(float* rec, uint32_t off, const float (&mo)[4]) {
+  rec4_t v;
+  v.x = __builtin_bit_cast(uint32_t, mo[0]);
+  v.y = __builtin_bit_cast(uint32_t, mo[1]);
+  v.z = __builtin_bit_cast(uint32_t, mo[2]);
+  v.w = __builtin_bit_cast(uint32_t, mo[3]);
+  __builtin_amdgcn_raw_buffer_store_b128(v, rsrc_at(rec), off, 0, AUX);
+}
+
+// The record of node n, read once (non-temporal).  A global load: this compiler lowers
+// __builtin_amdgcn_raw_buffer_load_b128 to a single-dword load and leaves the other three
+// components undefined (the b128 store above is lowered correctly, to buffer_store_dwordx4).
+typedef float rec4f_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void ld_rec_nt(const float* rec, uint32_t n, float (&mo)[4]) {
+  const rec4f_t v = __builtin_nontemporal_load(reinterpret_cast<const rec4f_t*>(rec) + n);
+  mo[0] = v.x;
+  mo[1] = v.y;
+  mo[2] = v.z;
+  mo[3] = v.w;
+}
+
 // Access to element (plane slot, node) of a state array: base + slot * plane + delta + off
 // with delta a uniform byte offset (the row shift of a link, |delta| < plane) and off the
 // lane's byte offset.  ADDR chooses how the uniform part reaches the hardware:

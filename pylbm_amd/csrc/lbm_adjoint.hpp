@@ -6,10 +6,12 @@
 namespace plbm {
 
 int adjoint_partials(const KArgs& a);
-// pattern: PYLBM_AB (lin -> lout), or PYLBM_AA_EVEN / PYLBM_AA_ODD in place (lin == lout)
+// pattern: PYLBM_AB (lin -> lout), or PYLBM_AA_EVEN / PYLBM_AA_ODD in place (lin == lout);
+// rec: f is the step's moment record (N x 4 fp32) instead of the stored state (part must be
+// nullptr)
 int launch_adjoint_step(int lattice, const KArgs& a, int coll, int storage, int lam_storage,
-                        int pattern, bool field, const void* f, const void* lin, void* lout,
-                        float* dF, double* part, hipStream_t s);
+                        int pattern, bool field, bool rec, const void* f, const void* lin,
+                        void* lout, float* dF, double* part, hipStream_t s);
 int launch_adjoint_seed(int lattice, const KArgs& a, int storage, int lam_storage, bool field,
                         const void* f, const float* G, void* lam, float* dF, hipStream_t s);
 int launch_adjoint_init(int lattice, int lam_storage, int N, const double* rho0,

@@ -145,6 +145,8 @@ struct KArgs {
   const void* links;   // per-node solid-source mask or nullptr
   const float* inlet;  // u_x(y) of the velocity inlet on the x = 0 wall, or nullptr
   int outlet;          // 1: pressure outlet on the last column
+  // moment records (k_step with REC, k_adjoint with REC): N x 4 fp32, node-major
+  float* rec;
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs, so workgroup b runs on XCD b % 8.
@@ -219,7 +221,7 @@ __host__ __device__ constexpr int pair_lo(int k) {
 // 128^2 cavity after 1000 steps: 2.3e-7 against 3.8e-7 in the pair form).
 template <class L, bool FORCE>
 __device__ __forceinline__ void collide_bgk(const float (&h)[L::Q], float (&d)[L::Q],
-                                            const float F[3], float r0) {
+                                            const float F[3], float r0, float (&mo)[4]) {
   constexpr int Q = L::Q;
   float drho = 0.f, j0 = 0.f, j1 = 0.f, j2 = 0.f;
 #pragma unroll
@@ -234,6 +236,10 @@ __device__ __forceinline__ void collide_bgk(const float (&h)[L::Q], float (&d)[L
   const float ux = (FORCE ? j0 + 0.5f * F[0] : j0) * inv;
   const float uy = (FORCE ? j1 + 0.5f * F[1] : j1) * inv;
   const float uz = (L::D == 3) ? (FORCE ? j2 + 0.5f * F[2] : j2) * inv : 0.0f;
+  mo[0] = rho;
+  mo[1] = ux;
+  mo[2] = uy;
+  mo[3] = uz;
   const float usq = ux * ux + uy * uy + uz * uz;
   const float uf = FORCE ? ux * F[0] + uy * F[1] + uz * F[2] : 0.0f;
   const float g = 1.0f - 0.5f * r0;
@@ -252,7 +258,7 @@ __device__ __forceinline__ void collide_bgk(const float (&h)[L::Q], float (&d)[L
 template <class L, int COLL, bool FORCE>
 __device__ __forceinline__ void collide_pairs(const float (&h)[L::Q], float (&d)[L::Q],
                                               const float F[3], float r0, float r1, float r2,
-                                              float r3) {
+                                              float r3, float (&mo)[4]) {
   // multiply-adds fuse within a statement only, never across statements, so the rounding
   // does not depend on the kernel this is inlined into (box or geometry kernel, one or
   // several nodes per lane, AB or AA pattern)
@@ -278,6 +284,10 @@ __device__ __forceinline__ void collide_pairs(const float (&h)[L::Q], float (&d)
   const float ux = (FORCE ? j0 + 0.5f * F[0] : j0) * inv;
   const float uy = (FORCE ? j1 + 0.5f * F[1] : j1) * inv;
   const float uz = (L::D == 3) ? (FORCE ? j2 + 0.5f * F[2] : j2) * inv : 0.0f;
+  mo[0] = rho;
+  mo[1] = ux;
+  mo[2] = uy;
+  mo[3] = uz;
   const float usq = ux * ux + uy * uy + uz * uz;
   const float uf = FORCE ? ux * F[0] + uy * F[1] + uz * F[2] : 0.0f;
   {
@@ -294,7 +304,7 @@ __device__ __forceinline__ void collide_pairs(const float (&h)[L::Q], float (&d)
     const float drho2 = 2.0f * drho, usq3 = 3.0f * usq;
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
-      const int i = pair_lo<L>(k), o = L::opp(i);
+      const int i = pair_lo<L>(k);
       const float cu = L::c(i, 0) * ux + L::c(i, 1) * uy + L::c(i, 2) * uz;
       const float w = (float)L::w(i);
       sn[k] = hs[k] - w * (drho2 + rho * (9.0f * cu * cu - usq3));
@@ -401,14 +411,25 @@ __device__ __forceinline__ void collide_pairs(const float (&h)[L::Q], float (&d)
   }
 }
 
+// mo receives the node's density and velocity as the collision used them: (rho, u_x, u_y,
+// u_z) with u = (sum c h + F/2) / rho (u_z = 0 in 2D); kernels that do not record them let
+// the compiler drop the copies.
+template <class L, int COLL, bool FORCE>
+__device__ __forceinline__ void collide_node(const float (&h)[L::Q], float (&d)[L::Q],
+                                             const float F[3], float r0, float r1, float r2,
+                                             float r3, float (&mo)[4]) {
+  if constexpr (COLL == PYLBM_BGK)
+    collide_bgk<L, FORCE>(h, d, F, r0, mo);
+  else
+    collide_pairs<L, COLL, FORCE>(h, d, F, r0, r1, r2, r3, mo);
+}
+
 template <class L, int COLL, bool FORCE>
 __device__ __forceinline__ void collide_node(const float (&h)[L::Q], float (&d)[L::Q],
                                              const float F[3], float r0, float r1, float r2,
                                              float r3) {
-  if constexpr (COLL == PYLBM_BGK)
-    collide_bgk<L, FORCE>(h, d, F, r0);
-  else
-    collide_pairs<L, COLL, FORCE>(h, d, F, r0, r1, r2, r3);
+  float mo[4];
+  collide_node<L, COLL, FORCE>(h, d, F, r0, r1, r2, r3, mo);
 }
 
 }  // namespace plbm

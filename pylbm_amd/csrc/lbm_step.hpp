@@ -83,7 +83,10 @@ __device__ __forceinline__ void store_node(const KArgs& a, const Lane& l, char* 
 // k * kBlock + t for k < NPL, so every wave access stays one contiguous, aligned run of 64
 // elements; the loads of all NPL nodes are issued before the first collision, which
 // doubles the bytes a wave keeps in flight when NPL = 2.
-template <class L, int COLL, int ST, int PAT, int FM, int ADDR, int NPL, int SP>
+// REC: also write the node's moment record (rho, u as the collision used them) to a.rec
+// (non-temporal: the records of a segment are read once, by its adjoint steps).
+template <class L, int COLL, int ST, int PAT, int FM, int ADDR, int NPL, int SP,
+          bool REC = false>
 __global__ void __launch_bounds__(kBlock)
 k_step(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T* B) {
   using C = Codec<ST>;
@@ -136,10 +139,11 @@ k_step(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T* B) 
   for (int k = 0; k < NPL; ++k) {
     float h[Q];  // deviations f - w of the pulled populations
     gather_finish<L, ST, PAT, ADDR>(l[k], Ab, plane, t[k], lid[k], h);
-    float d[Q];
-    collide_node<L, COLL, FM != kForceNone>(h, d, F[k], a.r0, a.r1, a.r2, a.r3);
+    float d[Q], mo[4];
+    collide_node<L, COLL, FM != kForceNone>(h, d, F[k], a.r0, a.r1, a.r2, a.r3, mo);
     if (k > 0 && !act[k]) break;
     store_node<L, ST, PAT, ADDR, SP>(a, l[k], Bb, plane, d);
+    if constexpr (REC) st_rec<2>(a.rec, l[k].n * 16u, mo);
   }
 }
 
@@ -172,7 +176,7 @@ inline KArgs with_npl(const KArgs& a, int npl) {
   return b;
 }
 
-template <class L, int COLL, int ST, int PAT, int ADDR, int NPL, int SP>
+template <class L, int COLL, int ST, int PAT, int ADDR, int NPL, int SP, bool REC = false>
 void launch_step_n(const KArgs& a0, int fmode, const void* src, void* dst, hipStream_t s) {
   using T = typename Codec<ST>::T;
   const KArgs a = with_npl(a0, NPL);
@@ -180,11 +184,12 @@ void launch_step_n(const KArgs& a0, int fmode, const void* src, void* dst, hipSt
   const T* in = (const T*)src;
   T* out = (T*)dst;
   if (fmode == kForceField)
-    k_step<L, COLL, ST, PAT, kForceField, ADDR, NPL, SP><<<grid, block, 0, s>>>(a, in, out);
+    k_step<L, COLL, ST, PAT, kForceField, ADDR, NPL, SP, REC><<<grid, block, 0, s>>>(a, in, out);
   else if (fmode == kForceUniform)
-    k_step<L, COLL, ST, PAT, kForceUniform, ADDR, NPL, SP><<<grid, block, 0, s>>>(a, in, out);
+    k_step<L, COLL, ST, PAT, kForceUniform, ADDR, NPL, SP, REC><<<grid, block, 0, s>>>(a, in,
+                                                                                      out);
   else
-    k_step<L, COLL, ST, PAT, kForceNone, ADDR, NPL, SP><<<grid, block, 0, s>>>(a, in, out);
+    k_step<L, COLL, ST, PAT, kForceNone, ADDR, NPL, SP, REC><<<grid, block, 0, s>>>(a, in, out);
 }
 
 // Several nodes per lane only pay on rows at least one full segment long; shorter rows
@@ -221,6 +226,17 @@ int launch_step_t(const KArgs& a, int fmode, const void* src, void* dst, hipStre
   const uint64_t plane = (uint64_t)a.N * sizeof(typename Codec<ST>::T);
   const uint64_t fbytes = (uint64_t)L::D * (uint64_t)a.N * sizeof(float);
   const int mode = fbytes < ((uint64_t)1 << 32) ? addr_mode(L::Q, plane) : 0;
+  if (a.rec) {
+    // moment records: the two-buffer pattern of the BGK / TRT rollouts, arrays under 4 GiB,
+    // one node per lane (the host checks the sizes)
+    if constexpr (PAT == PYLBM_AB && COLL != PYLBM_MRT) {
+      if (mode != 1) return PYLBM_ERANGE;
+      if (a.resident) launch_step_n<L, COLL, ST, PAT, 1, 1, 0, true>(a, fmode, src, dst, s);
+      else launch_step_n<L, COLL, ST, PAT, 1, 1, 2, true>(a, fmode, src, dst, s);
+      return (int)hipGetLastError();
+    }
+    return PYLBM_ENOTSUP;
+  }
   // a state resident in the Infinity Cache (always under 4 GiB) is stored with temporal
   // stores, so that the next launch finds it there; larger states stream past it
   switch (mode) {

@@ -138,6 +138,7 @@ def test_rollout_config_checks_adjoint_storage():
         RolloutConfig(p, 4, lam_storage="bf16")
     with pytest.raises(ValueError, match="BGK and TRT"):
         RolloutConfig(Problem((8, 8), tau=0.8, collision="mrt"), 4)
+    assert RolloutConfig(p, 4).records is None  # records unless tau is differentiated
 
 
 def test_lam16_scale_is_a_power_of_two_near_the_target():

@@ -267,3 +267,38 @@ def test_abi_rejects_fp32_side_planes_beyond_4gib():
     under = _params(storage=_lib.F16, nx=65536, ny=16383, n_nodes=65536 * 16383,
                     force_field=0x5000)
     assert lib.pylbm_hip_adjoint_workspace(ctypes.byref(under)) > 0
+
+
+def test_abi_rejects_bad_record_calls():
+    """The moment-record entries check their sizes, collision and geometry before any HIP
+    call: MRT and obstacles are not covered, and every array must stay under 4 GiB."""
+    lib = _lib.load()
+    a, b, c = ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000), ctypes.c_void_p(0x3000)
+    arr = (ctypes.c_void_p * 2)(0x5000, 0x6000)
+    which = ctypes.c_int(0)
+
+    def last():
+        return lib.pylbm_hip_last_error().decode()
+
+    mrt = _params(collision=_lib.MRT)
+    assert lib.pylbm_hip_run_record(ctypes.byref(mrt), a, arr, arr, 2, None) == 1003
+    assert "BGK and TRT" in last()
+    geo = _params(links=0x7000)
+    assert lib.pylbm_hip_run_record(ctypes.byref(geo), a, arr, arr, 2, None) == 1003
+    assert "box geometries" in last()
+    assert lib.pylbm_hip_adjoint_step_record(ctypes.byref(geo), a, b, c, None, None) == 1003
+    assert lib.pylbm_hip_run_record(ctypes.byref(_params(aa=1)), a, arr, arr, 2, None) == 1001
+    # D3Q19 512 x 512 x 256 with 16-bit states: the state is under 4 GiB, a q-plane fp32
+    # array (the adjoint states) is not
+    big = _params(lattice=_lib.D3Q19, storage=_lib.F16, nx=512, ny=512, nz=256,
+                  n_nodes=512 * 512 * 256)
+    assert lib.pylbm_hip_run_record(ctypes.byref(big), a, arr, arr, 2, None) == 1002
+    assert "4 GiB" in last()
+    assert lib.pylbm_hip_adjoint_step_record(ctypes.byref(big), a, b, c, None, None) == 1002
+    assert lib.pylbm_hip_adjoint_run_record(ctypes.byref(big), arr, 2, b, c, None,
+                                            ctypes.byref(which), None) == 1002
+    p = _params()
+    assert lib.pylbm_hip_adjoint_step_record(ctypes.byref(p), a, b, b, None, None) == 1001
+    assert lib.pylbm_hip_adjoint_run_record(ctypes.byref(p), arr, 2, b, b, None,
+                                            ctypes.byref(which), None) == 1001
+    assert lib.pylbm_hip_run_record(ctypes.byref(p), a, arr, arr, -1, None) == 1001

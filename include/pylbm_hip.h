@@ -77,6 +77,12 @@ typedef struct pylbm_params {
                            entries: PYLBM_F32 (0) or PYLBM_F16 (values rounded to nearest
                            fp16 on store, all arithmetic fp32); the forward entries ignore it.
                            No reference counterpart. */
+  const uint32_t* link_groups; /* NULL, or the group flags of pylbm_hip_link_groups for
+                           `links` (requires links): a 16-bit step then issues the bounce-back
+                           loads of every group that needs them together with its pull
+                           loads, and skips the mask load of the groups that do not (fp32
+                           steps ignore them).  Same results either way.  No reference
+                           counterpart. */
 } pylbm_params;
 
 /* ABI version (PYLBM_HIP_ABI_VERSION of the library). */
@@ -198,6 +204,17 @@ int pylbm_hip_reduce_noalloc(const pylbm_params* p, int kind, const void* f,
  * (pylbm/backends/torch_backend.py:80-103).
  */
 int pylbm_hip_links(const pylbm_params* p, const uint8_t* solid, void* links, void* stream);
+
+/*
+ * Group flags of a link mask: groups[g] (uint32) for the g-th group of 64 consecutive nodes
+ * of a row, g = (y + ny * z) * ceil(nx / 64) + x / 64, is nonzero when a node of the group has
+ * a solid pull source (links[n] != 0) or lies on a wall of a non-periodic x axis, else 0
+ * (groups: ny * nz * ceil(nx / 64) values).  Pass them in params.link_groups (beside
+ * params.links).  No reference counterpart: the reference has no fused obstacle kernel
+ * (pylbm/core.py:56-78 runs obstacles in its unfused solvers).
+ */
+int pylbm_hip_link_groups(const pylbm_params* p, const void* links, uint32_t* groups,
+                          void* stream);
 
 /*
  * Streaming copy of nbytes (multiple of 16) from src to dst: the measured copy ceiling.

@@ -34,7 +34,7 @@ EXPORTS = (
     "pylbm_hip_adjoint_init", "pylbm_hip_run_store", "pylbm_hip_adjoint_run",
     "pylbm_hip_links", "pylbm_hip_reduce_noalloc", "pylbm_hip_stream_probe",
     "pylbm_hip_plane_probe", "pylbm_hip_run_record", "pylbm_hip_adjoint_step_record",
-    "pylbm_hip_adjoint_run_record",
+    "pylbm_hip_adjoint_run_record", "pylbm_hip_link_groups",
 )
 
 
@@ -59,6 +59,7 @@ class Params(ctypes.Structure):
         ("inlet", ctypes.c_void_p),
         ("outlet", ctypes.c_int32),
         ("lam_storage", ctypes.c_int32),
+        ("link_groups", ctypes.c_void_p),
     ]
 
 
@@ -114,6 +115,7 @@ def load() -> Any:
         "pylbm_hip_adjoint_run": ([pp, ctypes.POINTER(vp), i64, vp, vp, vp, vp,
                                    ctypes.POINTER(ctypes.c_int), vp], i32),
         "pylbm_hip_links": ([pp, vp, vp, vp], i32),
+        "pylbm_hip_link_groups": ([pp, vp, vp, vp], i32),
         "pylbm_hip_run_record": ([pp, vp, ctypes.POINTER(vp), ctypes.POINTER(vp), i64, vp],
                                  i32),
         "pylbm_hip_adjoint_step_record": ([pp, vp, vp, vp, vp, vp], i32),

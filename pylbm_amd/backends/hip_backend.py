@@ -105,7 +105,7 @@ class HipFused(Solver):
                     problem.dim, -1), device=self.device)
         self.A = torch.empty(self.q * self.N, device=self.device, dtype=self.tdtype)
         self.B = None if self.aa else torch.empty_like(self.A)
-        self.links = self.inlet = None
+        self.links = self.link_groups = self.inlet = None
         if problem.inlet is not None:
             self.inlet = torch.as_tensor(np.asarray(problem.inlet, dtype=np.float32),
                                          device=self.device)
@@ -139,6 +139,7 @@ class HipFused(Solver):
             prm.rates[k] = r
         prm.force_field = _ptr(self.force_field)
         prm.links = _ptr(self.links)
+        prm.link_groups = _ptr(self.link_groups)
         prm.inlet = _ptr(self.inlet)
         prm.outlet = 1 if p.outlet else 0
         return prm
@@ -154,9 +155,17 @@ class HipFused(Solver):
         dt = torch.int16 if self.q == 9 else torch.int32
         links = torch.empty(self.N, dtype=dt, device=self.device)
         _lib.call("pylbm_hip_links", self.pref, _ptr(st), _ptr(links), _stream())
+        # one flag per 64-node group of a row: the waves of groups that touch a solid (or an
+        # x wall) load their bounce-back values with the pull, the others skip the mask
+        p = self.problem
+        nx = p.shape[0]
+        rows = self.N // nx
+        groups = torch.empty(rows * ((nx + 63) // 64), dtype=torch.int32, device=self.device)
+        _lib.call("pylbm_hip_link_groups", self.pref, _ptr(links), _ptr(groups), _stream())
         torch.cuda.current_stream().synchronize()
-        self.links = links
+        self.links, self.link_groups = links, groups
         self.params.links = _ptr(links)
+        self.params.link_groups = _ptr(groups)
 
     def set_force_field(self, field: Any) -> None:
         """Replace the per-cell force (d, *array_shape) without rebuilding the solver.

@@ -26,12 +26,19 @@ int launch_step_geo(const KArgs& a, int coll, int storage, int fmode, const void
                     void* dst, hipStream_t s);
 template <class L>
 int launch_links(const uint8_t* solid, void* links, int nx, int ny, int nz, hipStream_t s);
+template <class L>
+int launch_link_groups(const void* links, uint32_t* groups, int nx, int rows, int perx,
+                       hipStream_t s);
 extern template int launch_step_geo<D2Q9>(const KArgs&, int, int, int, const void*, void*,
                                           hipStream_t);
 extern template int launch_step_geo<D3Q19>(const KArgs&, int, int, int, const void*, void*,
                                            hipStream_t);
 extern template int launch_links<D2Q9>(const uint8_t*, void*, int, int, int, hipStream_t);
 extern template int launch_links<D3Q19>(const uint8_t*, void*, int, int, int, hipStream_t);
+extern template int launch_link_groups<D2Q9>(const void*, uint32_t*, int, int, int,
+                                             hipStream_t);
+extern template int launch_link_groups<D3Q19>(const void*, uint32_t*, int, int, int,
+                                              hipStream_t);
 }  // namespace plbm
 
 using namespace plbm;
@@ -97,6 +104,8 @@ int check_params(const pylbm_params* p, const char* what) {
   if (p->lam_storage != PYLBM_F32 && p->lam_storage != PYLBM_F16)
     return fail(PYLBM_EINVAL, "%s: lam_storage must be PYLBM_F32 or PYLBM_F16", what);
   if (p->outlet != 0 && p->outlet != 1) return fail(PYLBM_EINVAL, "%s: outlet must be 0 or 1", what);
+  if (p->link_groups && !p->links)
+    return fail(PYLBM_EINVAL, "%s: link_groups needs links", what);
   if (p->links || p->inlet || p->outlet) {
     if ((p->inlet || p->outlet) && p->lattice != PYLBM_D2Q9)
       return fail(PYLBM_ENOTSUP, "%s: the inlet and the outlet are D2Q9 only", what);
@@ -163,6 +172,8 @@ int make_kargs(const pylbm_params* p, KArgs& a, const char* what) {
   a.inlet = p->inlet;
   a.outlet = p->outlet;
   a.rec = nullptr;
+  a.lgroups = p->link_groups;
+  a.gpr = (p->nx + 63) / 64;
   return 0;
 }
 
@@ -420,6 +431,20 @@ int pylbm_hip_links(const pylbm_params* p, const uint8_t* solid, void* links, vo
   const int e = p->lattice == PYLBM_D2Q9 ? launch_links<D2Q9>(solid, links, p->nx, p->ny, p->nz, s)
                                          : launch_links<D3Q19>(solid, links, p->nx, p->ny, p->nz, s);
   return hip_fail(e, "pylbm_hip_links");
+}
+
+int pylbm_hip_link_groups(const pylbm_params* p, const void* links, uint32_t* groups,
+                          void* stream) {
+  if (int e = check_params(p, "pylbm_hip_link_groups")) return e;
+  if (!links || !groups) return fail(PYLBM_EINVAL, "pylbm_hip_link_groups: NULL buffer");
+  if (p->n_nodes == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  const int rows = p->ny * p->nz;
+  const int perx = p->periodic[0] ? 1 : 0;
+  const int e = p->lattice == PYLBM_D2Q9
+                    ? launch_link_groups<D2Q9>(links, groups, p->nx, rows, perx, s)
+                    : launch_link_groups<D3Q19>(links, groups, p->nx, rows, perx, s);
+  return hip_fail(e, "pylbm_hip_link_groups");
 }
 
 int pylbm_hip_copy(const void* src, void* dst, int64_t nbytes, void* stream) {

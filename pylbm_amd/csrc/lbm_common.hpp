@@ -147,6 +147,9 @@ struct KArgs {
   int outlet;          // 1: pressure outlet on the last column
   // moment records (k_step with REC, k_adjoint with REC): N x 4 fp32, node-major
   float* rec;
+  // group flags of the link mask (k_step_geo with GF): one word per 64-node group of a row
+  const uint32_t* lgroups;
+  int gpr;  // groups per row, ceil(nx / 64)
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs, so workgroup b runs on XCD b % 8.

@@ -35,18 +35,33 @@ __host__ __device__ constexpr uint32_t x_bits(int sign) {
   return m;
 }
 
+// Group flag of the wave that serves node slot l: the wave covers 64 consecutive nodes of
+// one row, aligned to 64 (x of its first active lane, over 64, names the group), so the
+// flag is uniform and read once per wave by a scalar load.
+__device__ __forceinline__ bool group_flag(const KArgs& a, const Lane& l) {
+  const int row = l.y + a.ny * l.z;
+  const int xg = __builtin_amdgcn_readfirstlane(l.x) >> 6;
+  return a.lgroups[row * a.gpr + xg] != 0u;
+}
+
 // Loads of one node: the pulled values (box rules), their additive terms, the mask.
-template <class L, int ST>
+// GF (group flags): gf says whether any lane of the wave has a solid source or an x wall;
+// if so, the own opp(i) slots of every lane are loaded here too (tw), with the pull loads,
+// and the choice between them waits for the mask; if not, neither the mask nor the own
+// slots are loaded.  Without GF the mask is loaded and the own slots follow it (geo_finish).
+template <class L, int ST, bool GF>
 __device__ __forceinline__ void geo_issue(const KArgs& a, const Lane& l, const char* Ab,
-                                          size_t plane, float uin,
+                                          size_t plane, float uin, bool gf,
                                           typename Codec<ST>::T (&t)[L::Q], float (&add)[L::Q],
-                                          uint32_t& bb) {
+                                          typename Codec<ST>::T (&tw)[L::Q], uint32_t& bb) {
   using T = typename Codec<ST>::T;
   using M = LinkMask<L>;
   constexpr int Q = L::Q;
   constexpr uint32_t TS = sizeof(T);
   bb = 0;
-  if (a.links) bb = (uint32_t)ld_at<M>(reinterpret_cast<const char*>(a.links), l.n * sizeof(M));
+  // without group flags the mask goes first: the own-slot loads of geo_finish wait for it
+  if (!GF && a.links)
+    bb = (uint32_t)ld_at<M>(reinterpret_cast<const char*>(a.links), l.n * sizeof(M));
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     const int cx = L::c(i, 0);
@@ -64,23 +79,32 @@ __device__ __forceinline__ void geo_issue(const KArgs& a, const Lane& l, const c
     if (L::D == 2 && cx == 1 && !r.wall)
       add[i] = l.wall_lo ? (float)(6.0 * L::w(i)) * uin : 0.0f;
   }
+  // with group flags the pull loads go first: they do not wait for the scalar flag load
+  if (GF && gf) {
+    bb = (uint32_t)ld_at<M>(reinterpret_cast<const char*>(a.links), l.n * sizeof(M));
+#pragma unroll
+    for (int i = 1; i < Q; ++i) tw[i] = ld_pl<T, 1>(Ab, plane, L::opp(i), 0, l.o0);
+  }
 }
 
 // Bounce-back of solid sources and x walls, decode, outlet: the deviations h of one node.
-template <class L, int ST>
+template <class L, int ST, bool GF>
 __device__ __forceinline__ void geo_finish(const KArgs& a, const Lane& l, const char* Ab,
                                            size_t plane, typename Codec<ST>::T (&t)[L::Q],
-                                           const float (&add)[L::Q], uint32_t bb,
+                                           const float (&add)[L::Q],
+                                           typename Codec<ST>::T (&tw)[L::Q], uint32_t bb,
                                            float (&h)[L::Q]) {
   using C = Codec<ST>;
   using T = typename C::T;
   constexpr int Q = L::Q;
   if (l.wall_lo) bb |= x_bits<L>(+1);
   if (l.wall_hi) bb |= x_bits<L>(-1);
+  // with GF a lane with bb != 0 belongs to a flagged group, whose own slots are loaded
   if (bb != 0) {
-    T tw[Q];
+    if (!GF) {
 #pragma unroll
-    for (int i = 1; i < Q; ++i) tw[i] = ld_pl<T, 1>(Ab, plane, L::opp(i), 0, l.o0);
+      for (int i = 1; i < Q; ++i) tw[i] = ld_pl<T, 1>(Ab, plane, L::opp(i), 0, l.o0);
+    }
 #pragma unroll
     for (int i = 1; i < Q; ++i)
       if ((bb >> i) & 1u) t[i] = tw[i];
@@ -112,8 +136,9 @@ __device__ __forceinline__ void geo_finish(const KArgs& a, const Lane& l, const 
   }
 }
 
-// NPL nodes per lane, one workgroup width apart, as k_step (lbm_step.hpp).
-template <class L, int COLL, int ST, int FM, int NPL>
+// NPL nodes per lane, one workgroup width apart, as k_step (lbm_step.hpp).  GF: the group
+// flags of a.lgroups select, per wave, the one-round gather of solid waves (geo_issue).
+template <class L, int COLL, int ST, int FM, int NPL, bool GF>
 __global__ void __launch_bounds__(kBlock)
 k_step_geo(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T* B) {
   using C = Codec<ST>;
@@ -137,13 +162,16 @@ k_step_geo(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T*
   char* Bb = reinterpret_cast<char*>(B);
   const float uin = a.inlet ? a.inlet[y] : 0.0f;  // uniform: one row per workgroup
 
-  T t[NPL][Q];
+  T t[NPL][Q], tw[NPL][Q];
   float add[NPL][Q];
   uint32_t bb[NPL];
   float F[NPL][3];
+  bool gf[NPL];
+#pragma unroll
+  for (int k = 0; k < NPL; ++k) gf[k] = GF ? group_flag(a, l[k]) : true;
 #pragma unroll
   for (int k = 0; k < NPL; ++k) {
-    geo_issue<L, ST>(a, l[k], Ab, plane, uin, t[k], add[k], bb[k]);
+    geo_issue<L, ST, GF>(a, l[k], Ab, plane, uin, gf[k], t[k], add[k], tw[k], bb[k]);
     F[k][0] = F[k][1] = F[k][2] = 0.0f;
     if (FM == 1) {
       F[k][0] = a.fx;
@@ -162,7 +190,7 @@ k_step_geo(const KArgs a, const typename Codec<ST>::T* A, typename Codec<ST>::T*
 #pragma unroll
   for (int k = 0; k < NPL; ++k) {
     float h[Q];
-    geo_finish<L, ST>(a, l[k], Ab, plane, t[k], add[k], bb[k], h);
+    geo_finish<L, ST, GF>(a, l[k], Ab, plane, t[k], add[k], tw[k], bb[k], h);
     float d[Q];
     collide_node<L, COLL, FM != 0>(h, d, F[k], a.r0, a.r1, a.r2, a.r3);
     if (k > 0 && !act[k]) break;
@@ -199,17 +227,34 @@ k_links(const uint8_t* solid, LinkMask<L>* links, int nx, int ny, int nz, int N)
 // 133.0k); the mask load and the bounce-back block shift the optimum away from the box
 // kernel's.  One for D3Q19, where the bounce-back block's extra registers make two nodes
 // per lane 6% slower (sphere-array box, 256^3, fp16).
-template <class L, int COLL, int ST, int NPL>
-int launch_geo_n(const KArgs& a0, int fmode, const void* src, void* dst, hipStream_t s) {
+template <class L, int COLL, int ST, int NPL, bool GF>
+void launch_geo_g(const KArgs& a, int fmode, const void* src, void* dst, hipStream_t s) {
   using T = typename Codec<ST>::T;
-  const KArgs a = with_npl(a0, NPL);
   const dim3 grid((unsigned)a.nblk), block(kBlock);
   if (fmode == 2)
-    k_step_geo<L, COLL, ST, 2, NPL><<<grid, block, 0, s>>>(a, (const T*)src, (T*)dst);
+    k_step_geo<L, COLL, ST, 2, NPL, GF><<<grid, block, 0, s>>>(a, (const T*)src, (T*)dst);
   else if (fmode == 1)
-    k_step_geo<L, COLL, ST, 1, NPL><<<grid, block, 0, s>>>(a, (const T*)src, (T*)dst);
+    k_step_geo<L, COLL, ST, 1, NPL, GF><<<grid, block, 0, s>>>(a, (const T*)src, (T*)dst);
   else
-    k_step_geo<L, COLL, ST, 0, NPL><<<grid, block, 0, s>>>(a, (const T*)src, (T*)dst);
+    k_step_geo<L, COLL, ST, 0, NPL, GF><<<grid, block, 0, s>>>(a, (const T*)src, (T*)dst);
+}
+
+// The group flags serve the 16-bit kernels, where a wave's bytes in flight are few and the
+// dependent second round of bounce-back loads cost most (A/B on MI355X with and without the
+// flags, two rounds each: sphere-array box 256^3 fp16 +7.5%; cylinder-array channel 4096^2
+// bf16 +2.6%, fp16 +0.6%).  The fp32 kernels keep the mask-first order: holding the own
+// slots of a flagged wave beside its pulled values in fp32 cost them 2.3% (D3Q19) and 0.7%
+// (D2Q9).
+template <class L, int COLL, int ST, int NPL>
+int launch_geo_n(const KArgs& a0, int fmode, const void* src, void* dst, hipStream_t s) {
+  const KArgs a = with_npl(a0, NPL);
+  if constexpr (sizeof(typename Codec<ST>::T) == 2) {
+    if (a.lgroups && a.links) {
+      launch_geo_g<L, COLL, ST, NPL, true>(a, fmode, src, dst, s);
+      return (int)hipGetLastError();
+    }
+  }
+  launch_geo_g<L, COLL, ST, NPL, false>(a, fmode, src, dst, s);
   return (int)hipGetLastError();
 }
 
@@ -250,6 +295,36 @@ int launch_step_geo(const KArgs& a, int coll, int storage, int fmode, const void
         return PYLBM_ENOTSUP;
   }
   return PYLBM_EINVAL;
+}
+
+// Group flags of a link mask: one wave per group of 64 consecutive nodes of a row; the word
+// is 1 when a node of the group has a solid source or sits on a wall of a non-periodic x.
+template <class L>
+__global__ void __launch_bounds__(256)
+k_link_groups(const LinkMask<L>* links, uint32_t* groups, int nx, int ngroups, int gpr,
+              int perx) {
+  const int g = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));  // uniform per wave
+  if (g >= ngroups) return;
+  const int row = g / gpr;
+  const int x = (g - row * gpr) * 64 + (int)(threadIdx.x & 63);
+  bool need = false;
+  if (x < nx) {
+    need = links[(size_t)row * nx + x] != 0;
+    if (!perx && (x == 0 || x == nx - 1)) need = true;
+  }
+  const uint64_t any = __ballot(need);
+  if ((threadIdx.x & 63) == 0) groups[g] = any != 0 ? 1u : 0u;
+}
+
+template <class L>
+int launch_link_groups(const void* links, uint32_t* groups, int nx, int rows, int perx,
+                       hipStream_t s) {
+  const int gpr = (nx + 63) / 64;
+  const int ngroups = rows * gpr;
+  const dim3 grid((unsigned)((ngroups + 3) / 4)), block(256);
+  k_link_groups<L><<<grid, block, 0, s>>>(reinterpret_cast<const LinkMask<L>*>(links), groups,
+                                          nx, ngroups, gpr, perx);
+  return (int)hipGetLastError();
 }
 
 template <class L>

@@ -148,6 +148,84 @@ def test_link_mask_equals_rolled_solid(shape):
     assert np.array_equal(got.reshape(ashape), want)
 
 
+@pytest.mark.parametrize("shape,periodic,node", [((29, 7), (True, True), (5, 3)),
+                                                 ((130, 5), (False, True), (90, 2)),
+                                                 ((200, 3, 2), (False, True, True), (100, 1, 0)),
+                                                 ((6, 5, 4), (True, True, True), (2, 2, 1))])
+def test_link_groups_flag_the_groups_that_need_bounce_back(shape, periodic, node):
+    """pylbm_hip_link_groups: one word per 64-node group of a row, nonzero exactly when a
+    node of the group has a solid pull source or sits on a wall of a non-periodic x (one
+    solid node: some groups flagged, others not)."""
+    ashape = tuple(reversed(shape))
+    solid = np.zeros(ashape, bool)
+    solid[tuple(reversed(node))] = True
+    s = _make("gather", Problem(shape, periodic=periodic, solid=solid))
+    nx = shape[0]
+    mask = s.links.cpu().numpy().astype(np.int64).reshape(-1, nx) & (2 ** s.q - 1)
+    gpr = (nx + 63) // 64
+    want = np.zeros((mask.shape[0], gpr), dtype=bool)
+    for j in range(gpr):
+        x0, x1 = 64 * j, min(64 * j + 64, nx)
+        want[:, j] = (mask[:, x0:x1] != 0).any(axis=1)
+        if not periodic[0] and (x0 == 0 or x1 == nx):
+            want[:, j] = True
+    got = s.link_groups.cpu().numpy().reshape(-1, gpr)
+    assert set(np.unique(got)) <= {0, 1}
+    assert np.array_equal(got != 0, want)
+    assert want.any() and not want.all()
+
+
+GROUP_CASES = {
+    "2d-walls-inlet-outlet": (dict(shape=(300, 40), inlet=True, outlet=True, collision="trt"),
+                              0.01),
+    # rows of 1100: four nodes per lane at 16-bit, a partial segment, the lid
+    "2d-wide-rows-lid": (dict(shape=(1100, 24), u_lid=0.05, collision="mrt"), 0.002),
+    "2d-periodic-force": (dict(shape=(1030, 16), periodic=(True, True), force=(1e-5, 0.0),
+                               collision="bgk"), 0.003),
+    "3d-spheres": (dict(shape=(96, 40, 24), periodic=(True, True, True),
+                        force=(1e-5, 0.0, 0.0), collision="trt"), None),
+    # walls on y and z (x periodic: an x wall would flag every group of its rows)
+    "3d-walls-lid": (dict(shape=(140, 12, 10), periodic=(True, False, False), u_lid=0.04,
+                          collision="bgk"), 0.001),
+}
+
+
+@pytest.mark.parametrize("variant", ["gather", "fused-f32s", "fused-fp16", "fused-bf16"])
+@pytest.mark.parametrize("name", list(GROUP_CASES))
+def test_group_flags_give_the_same_state(name, variant):
+    """With the group flags (the backend's default) the geometry kernel loads the bounce-back
+    values of flagged waves with the pull and skips the mask elsewhere; without them
+    (params.link_groups = NULL) it loads them after the mask.  The values and the arithmetic
+    are the same: equal states bit for bit."""
+    import torch
+
+    kw, frac = GROUP_CASES[name]
+    kw = dict(kw)
+    rng = np.random.default_rng(zlib.crc32(name.encode()))
+    shape = kw["shape"]
+    ashape = tuple(reversed(shape))
+    if frac is None:  # balls of radius 6 on a pitch of 16 nodes
+        c = [(np.arange(n) % 16) - 7.5 for n in ashape]
+        kw["solid"] = (c[0][:, None, None] ** 2 + c[1][None, :, None] ** 2
+                       + c[2][None, None, :] ** 2) <= 36.0
+    else:
+        kw["solid"] = rng.random(ashape) < frac
+    if kw.pop("inlet", False):
+        kw["inlet"] = 0.06 * rng.random(shape[1])
+    kw.setdefault("tau", 0.62)
+    p = Problem(**kw)
+    a, b = _make(variant, p), _make(variant, p)
+    assert a.params.link_groups and b.params.link_groups
+    groups = a.link_groups.cpu().numpy()
+    assert groups.any() and not groups.all()  # both paths of the kernel run
+    b.params.link_groups = None
+    a.step(30)
+    b.step(30)
+    torch.cuda.synchronize()
+    assert a.reduce("maxabs_diff", b) == 0.0
+    assert torch.equal(a.populations_device(), b.populations_device())
+
+
 @pytest.mark.parametrize("variant", ["fused", "fused-fp16"])
 def test_all_fluid_geometry_kernel_equals_box_kernel_4096(variant):
     """Full headline size: the geometry kernel with an all-fluid solid field gives the box

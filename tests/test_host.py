@@ -181,7 +181,9 @@ def test_params_struct_layout():
     assert _lib.Params.links.offset == 88
     assert _lib.Params.inlet.offset == 96
     assert _lib.Params.outlet.offset == 104
-    assert ctypes.sizeof(_lib.Params) == 112
+    assert _lib.Params.lam_storage.offset == 108
+    assert _lib.Params.link_groups.offset == 112
+    assert ctypes.sizeof(_lib.Params) == 120
 
 
 def _params(**over) -> _lib.Params:
@@ -302,3 +304,13 @@ def test_abi_rejects_bad_record_calls():
     assert lib.pylbm_hip_adjoint_run_record(ctypes.byref(p), arr, 2, b, b, None,
                                             ctypes.byref(which), None) == 1001
     assert lib.pylbm_hip_run_record(ctypes.byref(p), a, arr, arr, -1, None) == 1001
+
+
+def test_abi_rejects_link_groups_without_links():
+    lib = _lib.load()
+    a, b = ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000)
+    p = _params(link_groups=0x3000)
+    assert lib.pylbm_hip_step(ctypes.byref(p), a, b, _lib.AB, None) == 1001
+    assert "link_groups needs links" in lib.pylbm_hip_last_error().decode()
+    assert lib.pylbm_hip_link_groups(ctypes.byref(_params()), None, b, None) == 1001
+    assert lib.pylbm_hip_link_groups(ctypes.byref(_params()), a, None, None) == 1001

@@ -288,7 +288,10 @@ int pylbm_hip_adjoint_run(const pylbm_params* p, const void* const* states, int6
  * the steps' moment records (pylbm_hip_run_record) instead of the stored states: the same
  * adjoint algebra, one 16-byte load per node in place of the q-population pull.  No rate
  * sensitivities (they need the pulled populations): for rollouts that do not differentiate
- * tau.  Sizes as pylbm_hip_run_record. */
+ * tau.  Sizes as pylbm_hip_run_record.  On D3Q19 boxes periodic on every axis with rows of
+ * at most 128 nodes, fp32 adjoint states and no force field, pylbm_hip_adjoint_run_record
+ * runs two steps per launch, the first step's result held on chip, ping-ponging lam_a and
+ * lam_b; the result equals the single steps bit for bit. */
 int pylbm_hip_adjoint_step_record(const pylbm_params* p, const void* rec_t, const void* lam_in,
                                   void* lam_out, float* dF, void* stream);
 int pylbm_hip_adjoint_run_record(const pylbm_params* p, const void* const* recs, int64_t nsteps,

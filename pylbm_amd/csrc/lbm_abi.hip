@@ -543,11 +543,36 @@ static int adjoint_run_impl(const pylbm_params* p, const void* const* inputs, bo
   ar.reverse = 1;
   for (int64_t k = 0; k < nsteps; ++k)
     if (!inputs[k]) return fail(PYLBM_EINVAL, "%s: input %lld is NULL", what, (long long)k);
+  void* lin = lam_a;
+  void* lout = lam_b;
+  const bool field = p->force_field != nullptr;
+  if (rec && nsteps >= 2 &&
+      adjoint_pair_applies(a, p->lattice, p->collision, p->lam_storage, field)) {
+    // two steps per launch (lam_a -> lam_b -> lam_a ...), an odd last step two-buffer
+    int64_t k = nsteps - 1;
+    for (; k >= 1; k -= 2) {
+      if (int e = launch_adjoint_pair(a, p->collision, inputs[k], inputs[k - 1], lin, lout,
+                                      (hipStream_t)stream))
+        return hip_fail(e, what);
+      void* t = lin;
+      lin = lout;
+      lout = t;
+    }
+    if (k == 0) {
+      if (int e = launch_adjoint_step(p->lattice, a, p->collision, p->storage, p->lam_storage,
+                                      PYLBM_AB, field, true, inputs[0], lin, lout, dF,
+                                      dtau_part, (hipStream_t)stream))
+        return hip_fail(e, what);
+      void* t = lin;
+      lin = lout;
+      lout = t;
+    }
+    *which = (lin == lam_a) ? 0 : 1;
+    return 0;
+  }
   // The steps run in pairs in place in lam_a (the AA pattern of the adjoint, k_adjoint):
   // one adjoint-state array instead of two, which a state resident in the Infinity Cache
   // keeps there.  An odd last step goes lam_a -> lam_b.
-  void* lin = lam_a;
-  void* lout = lam_b;
   for (int64_t k = nsteps - 1; k >= 0; --k) {
     // the sweep direction alternates every step for resident states (remap_block)
     const bool rev = ((nsteps - 1 - k) & 1) && a.resident;

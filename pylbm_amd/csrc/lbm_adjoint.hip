@@ -13,6 +13,8 @@ extern template int adj_step_entry<D3Q19, float>(const KArgs&, int, int, int, bo
                                                  const void*, void*, float*, double*, hipStream_t);
 extern template int adj_step_entry<D3Q19, _Float16>(const KArgs&, int, int, int, bool, bool, const void*,
                                                     const void*, void*, float*, double*, hipStream_t);
+int adj_pair_d3q19(const KArgs& a, int coll, const void* recA, const void* recB,
+                   const void* lin, void* lout, hipStream_t s);
 }  // namespace adj
 
 namespace {
@@ -75,6 +77,16 @@ void init_lt(int lam_storage, dim3 grid, int N, const double* rho0, const double
 }  // namespace
 
 int adjoint_partials(const KArgs& a) { return 2 * a.nblk; }
+
+bool adjoint_pair_applies(const KArgs& a, int lattice, int coll, int lam_storage, bool field) {
+  return lattice == PYLBM_D3Q19 && (coll == PYLBM_BGK || coll == PYLBM_TRT) &&
+         lam_storage == PYLBM_F32 && !field && a.per == 7 && a.nx <= 128;
+}
+
+int launch_adjoint_pair(const KArgs& a, int coll, const void* recA, const void* recB,
+                        const void* lin, void* lout, hipStream_t s) {
+  return adj::adj_pair_d3q19(a, coll, recA, recB, lin, lout, s);
+}
 
 int launch_adjoint_step(int lattice, const KArgs& a, int coll, int storage, int lam_storage,
                         int pattern, bool field, bool rec, const void* f, const void* lin,

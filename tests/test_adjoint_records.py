@@ -291,3 +291,50 @@ def test_records_refuse_the_tau_gradient():
     tau = torch.tensor(0.8, dtype=torch.float64, device="cuda", requires_grad=True)
     with pytest.raises(ValueError, match="records"):
         rollout(cfg, tau=tau)
+
+
+PAIR_CASES = [((32, 32, 32), "bgk", 6, (0.0, 0.0, 0.0)),
+              ((32, 32, 32), "trt", 5, (1e-5, 0.0, 0.0)),    # odd: a last single step
+              ((40, 13, 10), "bgk", 4, (0.0, 0.0, 0.0)),     # partial y-tile, empty z-chunks
+              ((128, 8, 3), "trt", 2, (0.0, 2e-5, 0.0)),     # fewer planes than z-chunks
+              ((100, 6, 9), "bgk", 3, (0.0, 0.0, 0.0)),      # rows shorter than 128 lanes
+              ((128, 4, 1), "trt", 2, (0.0, 0.0, 0.0))]      # a single plane
+
+
+@pytest.mark.parametrize("shape,coll,n,force", PAIR_CASES)
+def test_pair_kernel_equals_single_record_steps(shape, coll, n, force):
+    """On D3Q19 boxes periodic on every axis (fp32 adjoint states, no force field),
+    pylbm_hip_adjoint_run_record takes two steps per launch, holding the first step's
+    result in LDS; it equals single two-buffer record steps bit for bit."""
+    from pylbm_amd import _lib
+    from pylbm_amd.adjoint import RolloutConfig, _Engine, _ptr_array
+
+    rng = np.random.default_rng(13)
+    ashape = tuple(reversed(shape))
+    p = Problem(shape, tau=0.7, periodic=(True, True, True), collision=coll, force=force)
+    q, N = p.lattice.q, p.n_nodes
+    eng = _Engine(RolloutConfig(p, 1, storage="f16"), None, 0.7)
+    f0 = eng.state()
+    u0 = torch.as_tensor(0.02 * rng.standard_normal((3,) + ashape), device="cuda")
+    eng.init(None, u0, f0)
+    states = [eng.state() for _ in range(n)]
+    recs = [eng.record() for _ in range(n)]
+    _lib.call("pylbm_hip_run_record", eng.ref, f0.data_ptr(), _ptr_array(states),
+              _ptr_array(recs), n, None)
+    lam0 = torch.as_tensor(0.01 * rng.standard_normal(q * N), dtype=torch.float32,
+                           device="cuda")
+    which = ctypes.c_int(0)
+    lam = [lam0.clone(), torch.full_like(lam0, float("nan"))]
+    _lib.call("pylbm_hip_adjoint_run_record", eng.ref, _ptr_array(recs), n, lam[0].data_ptr(),
+              lam[1].data_ptr(), None, ctypes.byref(which), None)
+    got = lam[which.value]
+    ref = [lam0.clone(), torch.full_like(lam0, float("nan"))]
+    cur = 0
+    for k in range(n - 1, -1, -1):
+        ref[1 - cur].fill_(float("nan"))
+        _lib.call("pylbm_hip_adjoint_step_record", eng.ref, recs[k].data_ptr(),
+                  ref[cur].data_ptr(), ref[1 - cur].data_ptr(), None, None)
+        cur = 1 - cur
+    torch.cuda.synchronize()
+    assert torch.isfinite(got).all(), "every adjoint slot is written once"
+    assert torch.equal(got, ref[cur])

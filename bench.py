@@ -8,7 +8,8 @@ shear wave u_x = 0.01 sin(2 pi y / 4096), u_y = 0, rho = 1 + 1e-3 U(-1, 1) per c
 seeded generator.  Populations are resident in HBM before timing; after a load phase of
 ``--warm-time`` seconds and ``warmup`` untimed steps, exactly ``steps`` launches are timed
 with HIP events on the launch stream (median of three such windows).  The same line carries
-every other variant of that configuration (TRT, MRT, AA, fp16/bf16 shifted storage), the
+every other variant of that configuration (TRT, MRT, AA, fp16/bf16 shifted storage, two
+steps per launch), the
 D3Q19 512^3 TRT Taylor-Green runs (fp32, fp16, bf16; pull and AA), the geometry kernels,
 the forward+adjoint rollouts, the measured copy ceiling, the roofline of the dominant
 kernel, the north-star ratios of this run and the float64 NumPy oracle timed on the host
@@ -17,8 +18,9 @@ kernel, the north-star ratios of this run and the float64 NumPy oracle timed on 
 step count, each at least ``--min-time`` seconds, median reported, with the GPU's clocks,
 power, temperatures and limiter counters recorded per leg.
 
-``--profile [d2q9|d3q19|d3q19-fp16|d3q19-aa|d3q19-aa-fp16|geometry-d2q9|geometry-d3q19]`` runs
-only one step kernel (the headline by default) and the copy kernel (for rocprofv3);
+``--profile [d2q9|d2q9-2step|d2q9-2step-fp16|d3q19|d3q19-fp16|d3q19-aa|d3q19-aa-fp16|geometry-d2q9|
+geometry-d3q19]`` runs only one step kernel (the headline by default) and the copy kernel (for
+rocprofv3);
 ``--profile copy`` runs every probe of the copy ceiling over 4 GiB buffers.
 """
 
@@ -332,10 +334,12 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--quick", action="store_true", help="headline + copy ceiling only")
     ap.add_argument("--profile", nargs="?", const="d2q9", default=None,
-                    choices=["d2q9", "d3q19", "d3q19-fp16", "d3q19-aa", "d3q19-aa-fp16",
+                    choices=["d2q9", "d2q9-2step", "d2q9-2step-fp16", "d3q19", "d3q19-fp16",
+                             "d3q19-aa", "d3q19-aa-fp16",
                              "geometry-d2q9", "geometry-d3q19", "copy", "planes"],
                     help="only one step kernel and the copy kernel, for rocprofv3: the headline "
-                         "(d2q9, default), D3Q19 512^3 TRT fp32 / fp16, two-buffer or AA, or "
+                         "(d2q9, default), its two-steps-per-launch form (fp32 / fp16), D3Q19 "
+                         "512^3 TRT fp32 / fp16, two-buffer or AA, or "
                          "the geometry kernel (fp16) on the cylinder-array channel 4096^2 / "
                          "the sphere-array box 256^3")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -426,9 +430,11 @@ def main() -> int:
         from pylbm_amd import backends as B
         from pylbm_amd.problem import Problem
 
-        if a.profile == "d2q9":
+        if a.profile.startswith("d2q9"):
             p = problem_2d("bgk", n, rho, u)
-            s = B.make("hip", "fused", p, init="none")
+            v = {"d2q9": "fused", "d2q9-2step": "fused-2step",
+                 "d2q9-2step-fp16": "fused-2step-fp16"}[a.profile]
+            s = B.make("hip", v, p, init="none")
             s.init_equilibrium(rho, u)
             nbytes = 9 * n * n * 4  # the bytes one step reads (and writes)
         elif a.profile.startswith("geometry"):
@@ -490,7 +496,8 @@ def main() -> int:
     if not a.quick:
         for coll in ("bgk", "trt", "mrt"):
             for v in ("fused", "fused-f32s", "fused-aa", "fused-fp16", "fused-bf16",
-                      "fused-aa-fp16", "fused-aa-bf16"):
+                      "fused-aa-fp16", "fused-aa-bf16", "fused-2step", "fused-2step-fp16",
+                      "fused-2step-bf16"):
                 if coll != "bgk" and v == "fused-f32s":
                     continue
                 r = run_variant(v, problem_2d(coll, n, rho, u), (rho, u), proto)

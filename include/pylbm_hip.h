@@ -33,7 +33,7 @@ extern "C" {
 enum { PYLBM_D2Q9 = 0, PYLBM_D3Q19 = 1 };
 enum { PYLBM_BGK = 0, PYLBM_TRT = 1, PYLBM_MRT = 2 };
 enum { PYLBM_F32 = 0, PYLBM_F16 = 1, PYLBM_BF16 = 2, PYLBM_F32S = 3 };
-enum { PYLBM_AB = 0, PYLBM_AA_ODD = 1, PYLBM_AA_EVEN = 2 };
+enum { PYLBM_AB = 0, PYLBM_AA_ODD = 1, PYLBM_AA_EVEN = 2, PYLBM_AB2 = 3 };
 enum {
   PYLBM_RED_MASS = 0, PYLBM_RED_KINETIC = 1, PYLBM_RED_MAXABS_DIFF = 2, PYLBM_RED_LOSS = 3,
   PYLBM_RED_DOT = 4
@@ -95,6 +95,10 @@ const char* pylbm_hip_last_error(void);
  * One fused time step: pull streaming + bounce-back + Ladd lid, moments, equilibrium,
  * Guo forcing, BGK/TRT/MRT collision, store.  pattern = PYLBM_AB reads src writes dst
  * (src != dst); PYLBM_AA_ODD / PYLBM_AA_EVEN work in place (src == dst required).
+ * PYLBM_AB2 advances two steps in one launch, src -> dst, with the intermediate state on
+ * chip (the same values as two PYLBM_AB steps); D2Q9 boxes periodic on both axes whose
+ * state is under 4 GiB only (PYLBM_ENOTSUP otherwise).  No reference counterpart for
+ * PYLBM_AB2.
  * Replaces FusedSolver._launch (pylbm/backends/_fused.py:108-110,
  * pylbm/backends/cupy_backend.py:101-103) and the kernel lbm_kernel
  * (pylbm/backends/cuda/lbm.cu:50-53).
@@ -110,6 +114,15 @@ int pylbm_hip_step(const pylbm_params* p, const void* src, void* dst, int patter
  * (pylbm/backends/base.py:68-73).
  */
 int pylbm_hip_run(const pylbm_params* p, void* a, void* b, int64_t nsteps, void* stream);
+
+/*
+ * nsteps (even) time steps as nsteps / 2 launches of the PYLBM_AB2 pattern, ping-ponging
+ * a -> b -> a ...; after an odd nsteps / 2 the state is in b.  D2Q9 boxes periodic on both
+ * axes with a state under 4 GiB (PYLBM_ENOTSUP otherwise).  The step loop of the
+ * two-steps-per-launch variants; no reference counterpart (a form of
+ * FusedSolver._advance, pylbm/backends/_fused.py:116-124).
+ */
+int pylbm_hip_run_pairs(const pylbm_params* p, void* a, void* b, int64_t nsteps, void* stream);
 
 /*
  * nsteps two-buffer steps from src, writing the state after step k to dst[k] (a host array

@@ -17,13 +17,13 @@ ABI_VERSION = 4
 D2Q9, D3Q19 = 0, 1
 BGK, TRT, MRT = 0, 1, 2
 F32, F16, BF16, F32S = 0, 1, 2, 3
-AB, AA_ODD, AA_EVEN = 0, 1, 2
+AB, AA_ODD, AA_EVEN, AB2 = 0, 1, 2, 3
 RED_MASS, RED_KINETIC, RED_MAXABS_DIFF, RED_LOSS, RED_DOT = 0, 1, 2, 3, 4
 
 LATTICE_IDS = {"D2Q9": D2Q9, "D3Q19": D3Q19}
 COLLISION_IDS = {"bgk": BGK, "trt": TRT, "mrt": MRT}
 STORAGE_IDS = {"f32": F32, "f16": F16, "bf16": BF16, "f32s": F32S}
-PATTERN_IDS = {"ab": AB, "aa-odd": AA_ODD, "aa-even": AA_EVEN}
+PATTERN_IDS = {"ab": AB, "aa-odd": AA_ODD, "aa-even": AA_EVEN, "ab2": AB2}
 
 #: every symbol include/pylbm_hip.h declares
 EXPORTS = (
@@ -34,7 +34,7 @@ EXPORTS = (
     "pylbm_hip_adjoint_init", "pylbm_hip_run_store", "pylbm_hip_adjoint_run",
     "pylbm_hip_links", "pylbm_hip_reduce_noalloc", "pylbm_hip_stream_probe",
     "pylbm_hip_plane_probe", "pylbm_hip_run_record", "pylbm_hip_adjoint_step_record",
-    "pylbm_hip_adjoint_run_record", "pylbm_hip_link_groups",
+    "pylbm_hip_adjoint_run_record", "pylbm_hip_link_groups", "pylbm_hip_run_pairs",
 )
 
 
@@ -98,6 +98,7 @@ def load() -> Any:
         "pylbm_hip_last_error": ([], ctypes.c_char_p),
         "pylbm_hip_step": ([pp, vp, vp, i32, vp], i32),
         "pylbm_hip_run": ([pp, vp, vp, i64, vp], i32),
+        "pylbm_hip_run_pairs": ([pp, vp, vp, i64, vp], i32),
         "pylbm_hip_encode": ([pp, vp, vp, vp], i32),
         "pylbm_hip_decode": ([pp, vp, vp, vp], i32),
         "pylbm_hip_init_equilibrium": ([pp, vp, vp, vp, vp], i32),

@@ -26,6 +26,9 @@ _TABLE: dict[tuple[str, str], tuple[str, str]] = {
     ("hip", "fused-aa-bf16"): (_HIP, "HipFusedAABF16"),
     ("hip", "gather"): (_HIP, "HipGather"),
     ("hip", "fused-f32s"): (_HIP, "HipFusedF32S"),
+    ("hip", "fused-2step"): (_HIP, "HipFused2Step"),
+    ("hip", "fused-2step-fp16"): (_HIP, "HipFused2StepFP16"),
+    ("hip", "fused-2step-bf16"): (_HIP, "HipFused2StepBF16"),
 }
 
 COLORS = {"hip": "#e4002b"}

@@ -66,6 +66,8 @@ class HipFused(Solver):
     description = "hand-written HIP (gfx950), fused pull, two buffers, fp32"
     storage: ClassVar[str] = "f32"
     aa: ClassVar[bool] = False
+    #: two time steps per launch (the intermediate state on chip; periodic D2Q9 boxes)
+    pair: ClassVar[bool] = False
     lattices: ClassVar[tuple[str, ...]] = ("D2Q9", "D3Q19")
     collisions: ClassVar[tuple[str, ...]] = ("bgk", "trt", "mrt")
 
@@ -84,6 +86,10 @@ class HipFused(Solver):
                 return False, "with obstacles, inlet or outlet the state must be < 4 GiB"
         if problem.lattice.name not in cls.lattices:
             return False, f"{problem.lattice.name} not implemented"
+        if cls.pair and not (problem.is_box() and all(problem.periodic)):
+            return False, "two steps per launch run on boxes periodic on every axis"
+        if cls.pair and problem.lattice.q * problem.n_nodes * item >= 2**32:
+            return False, "two steps per launch need the state under 4 GiB"
         if problem.collision not in cls.collisions:
             return False, f"{problem.collision} not implemented"
         if problem.n_nodes > MAX_NODES or problem.n_nodes * item >= 2**32:
@@ -92,7 +98,7 @@ class HipFused(Solver):
 
     def __init__(self, problem: Problem, init: str = "auto") -> None:
         super().__init__(problem)
-        self.step_multiple = 2 if self.aa else 1
+        self.step_multiple = 2 if (self.aa or self.pair) else 1
         self.itemsize = 2 if self.storage in ("f16", "bf16") else 4
         self.q = problem.lattice.q
         self.N = problem.n_nodes
@@ -257,15 +263,22 @@ class HipFused(Solver):
 
     # ------------------------------------------------------------------ stepping
     def launch(self, pattern: str) -> None:
-        """One kernel launch: 'ab' (A -> B, then swap), 'aa-odd' or 'aa-even' (in place)."""
+        """One kernel launch: 'ab' (A -> B, then swap), 'ab2' (two steps A -> B, then swap;
+        periodic D2Q9 boxes), 'aa-odd' or 'aa-even' (in place)."""
         pid = _lib.PATTERN_IDS[pattern]
-        if pid == _lib.AB:
+        if pid in (_lib.AB, _lib.AB2):
             _lib.call("pylbm_hip_step", self.pref, _ptr(self.A), _ptr(self.B), pid, _stream())
             self.A, self.B = self.B, self.A
         else:
             _lib.call("pylbm_hip_step", self.pref, _ptr(self.A), _ptr(self.A), pid, _stream())
 
     def _advance(self, n: int) -> None:
+        if self.pair:
+            _lib.call("pylbm_hip_run_pairs", self.pref, _ptr(self.A), _ptr(self.B), int(n),
+                      _stream())
+            if (n // 2) % 2:
+                self.A, self.B = self.B, self.A
+            return
         _lib.call("pylbm_hip_run", self.pref, _ptr(self.A), _ptr(self.B), int(n), _stream())
         if not self.aa and n % 2:
             self.A, self.B = self.B, self.A
@@ -324,3 +337,38 @@ class HipFusedAABF16(HipFused):
     description = "hand-written HIP (gfx950), AA pattern + bf16 storage of f - w_i"
     storage = "bf16"
     aa = True
+
+
+class HipFused2Step(HipFused):
+    """Two time steps per launch on D2Q9 boxes periodic on both axes: the first step's
+    populations stay in LDS, so the state crosses memory once per two steps (no reference
+    counterpart; the values are those of two ``fused`` steps)."""
+
+    variant = "fused-2step"
+    description = ("hand-written HIP (gfx950), fused pull, two steps per launch, state between "
+                   "them on chip, fp32")
+    pair = True
+    lattices = ("D2Q9",)
+
+    @property
+    def bytes_per_node(self) -> int:
+        """Minimal traffic of one node update: the state is read once and written once per
+        two steps, so q populations cross memory per update."""
+        return self.problem.lattice.q * self.itemsize
+
+    def memory_bytes(self) -> int:
+        return 2 * self.problem.lattice.q * self.itemsize * self.problem.n_nodes
+
+
+class HipFused2StepFP16(HipFused2Step):
+    variant = "fused-2step-fp16"
+    description = ("hand-written HIP (gfx950), fused pull, two steps per launch, fp16 storage "
+                   "of f - w_i")
+    storage = "f16"
+
+
+class HipFused2StepBF16(HipFused2Step):
+    variant = "fused-2step-bf16"
+    description = ("hand-written HIP (gfx950), fused pull, two steps per launch, bf16 storage "
+                   "of f - w_i")
+    storage = "bf16"

@@ -14,6 +14,8 @@
 #include "lbm_step.hpp"
 
 namespace plbm {
+int launch_step_pair_d2q9(const KArgs& a, int coll, int storage, int fmode, const void* src,
+                          void* dst, hipStream_t s);
 template <class L>
 int launch_step(const KArgs& a, int coll, int storage, int pattern, int fmode,
                 const void* src, void* dst, hipStream_t s);
@@ -177,11 +179,23 @@ int make_kargs(const pylbm_params* p, KArgs& a, const char* what) {
   return 0;
 }
 
+// the two-steps-per-launch pattern: D2Q9 boxes periodic on both axes, the state under 4 GiB
+// (the kernel addresses it through one buffer descriptor)
+bool pair_steps_apply(const pylbm_params* p) {
+  const uint64_t ts = (p->storage == PYLBM_F16 || p->storage == PYLBM_BF16) ? 2 : 4;
+  return p->lattice == PYLBM_D2Q9 && p->periodic[0] && p->periodic[1] && !p->links &&
+         !p->inlet && !p->outlet && !p->aa && 9ull * (uint64_t)p->n_nodes * ts < (1ull << 32);
+}
+
 int step_impl(const pylbm_params* p, const KArgs& a, const void* src, void* dst, int pattern,
               hipStream_t s) {
   const int fmode = p->force_field != nullptr ? kForceField
                     : (a.fx != 0.0f || a.fy != 0.0f || a.fz != 0.0f) ? kForceUniform
                                                                      : kForceNone;
+  if (pattern == PYLBM_AB2) {
+    if (!pair_steps_apply(p)) return PYLBM_ENOTSUP;
+    return launch_step_pair_d2q9(a, p->collision, p->storage, fmode, src, dst, s);
+  }
   if (has_geometry(p)) {
     if (pattern != PYLBM_AB) return PYLBM_ENOTSUP;
     if (p->lattice == PYLBM_D2Q9)
@@ -261,9 +275,12 @@ int pylbm_hip_step(const pylbm_params* p, const void* src, void* dst, int patter
                    void* stream) {
   if (int e = check_params(p, "pylbm_hip_step")) return e;
   if (!src || !dst) return fail(PYLBM_EINVAL, "pylbm_hip_step: NULL buffer");
-  if (pattern == PYLBM_AB) {
+  if (pattern == PYLBM_AB || pattern == PYLBM_AB2) {
     if (src == dst) return fail(PYLBM_EINVAL, "pylbm_hip_step: AB needs src != dst");
     if (p->aa) return fail(PYLBM_EINVAL, "pylbm_hip_step: AB pattern on an AA-layout state");
+    if (pattern == PYLBM_AB2 && !pair_steps_apply(p))
+      return fail(PYLBM_ENOTSUP, "pylbm_hip_step: two steps per launch run on D2Q9 boxes "
+                  "periodic on both axes with a state under 4 GiB only");
   } else if (pattern == PYLBM_AA_ODD || pattern == PYLBM_AA_EVEN) {
     if (src != dst) return fail(PYLBM_EINVAL, "pylbm_hip_step: AA patterns work in place");
   } else {
@@ -307,6 +324,30 @@ int pylbm_hip_run(const pylbm_params* p, void* a_buf, void* b_buf, int64_t nstep
   for (int64_t k = 0; k < nsteps; ++k) {
     if (int e = step_impl(p, ((k & 1) && alternate) ? ar : a, src, dst, PYLBM_AB, s))
       return hip_fail(e, "pylbm_hip_run");
+    void* t = src;
+    src = dst;
+    dst = t;
+  }
+  return 0;
+}
+
+int pylbm_hip_run_pairs(const pylbm_params* p, void* a_buf, void* b_buf, int64_t nsteps,
+                        void* stream) {
+  if (int e = check_params(p, "pylbm_hip_run_pairs")) return e;
+  if (nsteps < 0 || nsteps % 2)
+    return fail(PYLBM_EINVAL, "pylbm_hip_run_pairs: nsteps must be even and >= 0");
+  if (!a_buf || !b_buf || a_buf == b_buf)
+    return fail(PYLBM_EINVAL, "pylbm_hip_run_pairs: needs two distinct buffers");
+  if (!pair_steps_apply(p))
+    return fail(PYLBM_ENOTSUP, "pylbm_hip_run_pairs: D2Q9 boxes periodic on both axes with a "
+                "state under 4 GiB only");
+  KArgs a;
+  if (int e = make_kargs(p, a, "pylbm_hip_run_pairs")) return e;
+  void* src = a_buf;
+  void* dst = b_buf;
+  for (int64_t k = 0; k < nsteps; k += 2) {
+    if (int e = step_impl(p, a, src, dst, PYLBM_AB2, (hipStream_t)stream))
+      return hip_fail(e, "pylbm_hip_run_pairs");
     void* t = src;
     src = dst;
     dst = t;

@@ -225,6 +225,9 @@ __host__ __device__ constexpr int pair_lo(int k) {
 template <class L, bool FORCE>
 __device__ __forceinline__ void collide_bgk(const float (&h)[L::Q], float (&d)[L::Q],
                                             const float F[3], float r0, float (&mo)[4]) {
+  // multiply-adds fuse within a statement only (as in collide_pairs), so the rounding does
+  // not depend on the kernel this is inlined into
+#pragma clang fp contract(on)
   constexpr int Q = L::Q;
   float drho = 0.f, j0 = 0.f, j1 = 0.f, j2 = 0.f;
 #pragma unroll

@@ -55,6 +55,9 @@ RUNG_LABELS: dict[tuple[str, str], str] = {
     ("hip", "fused-bf16"): "HIP fused bf16",
     ("hip", "fused-aa-fp16"): "HIP AA fp16",
     ("hip", "fused-aa-bf16"): "HIP AA bf16",
+    ("hip", "fused-2step"): "HIP 2 steps/launch",
+    ("hip", "fused-2step-fp16"): "HIP 2 steps/launch fp16",
+    ("hip", "fused-2step-bf16"): "HIP 2 steps/launch bf16",
 }
 
 #: bar colours per backend
@@ -453,7 +456,8 @@ def _save(fig, out_dir: Path, name: str, formats: Iterable[str]) -> list[Path]:
 #: hip rungs share a backend, so their figures colour by variant
 HIP_COLOURS = {"gather": "#8c564b", "fused": "#d62728", "fused-f32s": "#ff9896",
                "fused-aa": "#ff7f0e", "fused-fp16": "#1f77b4", "fused-bf16": "#9467bd",
-               "fused-aa-fp16": "#2ca02c", "fused-aa-bf16": "#17becf"}
+               "fused-aa-fp16": "#2ca02c", "fused-aa-bf16": "#17becf", "fused-2step": "#7f7f7f",
+               "fused-2step-fp16": "#bcbd22", "fused-2step-bf16": "#e377c2"}
 
 
 def _colour(key: Rung) -> str:

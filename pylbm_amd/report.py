@@ -25,9 +25,11 @@ import re
 from pathlib import Path
 from typing import Any, Iterable
 
-#: hip variants in ladder order: two-buffer before AA, fp32 before 16-bit storage
+#: hip variants in ladder order: two-buffer before AA, fp32 before 16-bit storage, one step
+#: per launch before two
 HIP_ORDER = ["gather", "fused", "fused-f32s", "fused-aa", "fused-fp16", "fused-bf16",
-             "fused-aa-fp16", "fused-aa-bf16"]
+             "fused-aa-fp16", "fused-aa-bf16", "fused-2step", "fused-2step-fp16",
+             "fused-2step-bf16"]
 
 
 def load(path: str | Path) -> dict:

@@ -7,7 +7,7 @@
   ``torch.empty`` destination the caching allocator can instead return a buffer holding an
   earlier valid state and hide a missing store).  The C loops (pylbm_hip_run_store, with
   the reversed sweeps of resident states) are covered the same way, one fresh NaN buffer
-  per step.
+  per step.  The two-steps-per-launch kernel is covered the same way, per launch.
 * AA steps work in place, so their destinations cannot be poisoned.  Instead each launch
   runs on X and on X + eps g, g a constant ghost field that changes no gathered moment:
   with BGK at tau = 1 the two outputs agree to rounding, while a slot the launch did not
@@ -48,6 +48,15 @@ SMALL = {
 }
 TWO_BUFFER = ["fused", "fused-f32s", "fused-fp16", "fused-bf16"]
 AA = ["fused-aa", "fused-aa-fp16", "fused-aa-bf16"]
+PAIR = ["fused-2step", "fused-2step-fp16", "fused-2step-bf16"]
+PAIR_CASES = {
+    "2d-periodic-mrt": SMALL["2d-periodic-mrt"],
+    # a partial last tile of columns, chunks of 9 rows with a shorter last one, a force
+    "2d-partial-tile": dict(shape=(600, 71), tau=0.62, collision="bgk", periodic=(True, True),
+                            force=(1e-5, 0.0)),
+    # two chunks of rows, the last one row high
+    "2d-one-row-chunk": dict(shape=(300, 10), tau=0.7, collision="trt", periodic=(True, True)),
+}
 
 
 def _geometry_case() -> dict:
@@ -97,6 +106,23 @@ def test_two_buffer_step_writes_every_slot(name, variant):
         n_bad = int((~torch.isfinite(poisoned.A.float())).sum())
         assert n_bad == 0, f"{name} hip/{variant} step {k}: {n_bad} slots left unwritten"
         assert torch.equal(poisoned.A, clean.A), f"{name} hip/{variant} step {k}"
+
+
+@pytest.mark.parametrize("variant", PAIR)
+@pytest.mark.parametrize("name", list(PAIR_CASES))
+def test_two_step_launch_writes_every_slot(name, variant):
+    """Two steps per launch (the intermediate state on chip): every destination slot is
+    written once, with the value the launch computes into a zero-filled destination."""
+    poisoned, clean = _pair(variant, PAIR_CASES[name])
+    for k in range(3):
+        poisoned.B.fill_(NAN)
+        clean.B.zero_()
+        poisoned.launch("ab2")
+        clean.launch("ab2")
+        torch.cuda.synchronize()
+        n_bad = int((~torch.isfinite(poisoned.A.float())).sum())
+        assert n_bad == 0, f"{name} hip/{variant} launch {k}: {n_bad} slots left unwritten"
+        assert torch.equal(poisoned.A, clean.A), f"{name} hip/{variant} launch {k}"
 
 
 @pytest.mark.parametrize("name", ["2d-periodic-mrt", "2d-short-rows", "3d-periodic",

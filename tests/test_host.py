@@ -118,7 +118,7 @@ def test_initial_populations_match_oracle():
 
 def test_registry_never_crashes():
     st = B.status()
-    assert len(st) == len(B.entries()) == 8
+    assert len(st) == len(B.entries()) == 11
     for b, v, ok, why in st:
         assert b == "hip"
         assert ok or why  # unavailable entries say why
@@ -134,7 +134,8 @@ def test_supports_geometry_on_two_buffer_variants_only():
     geo = Problem((8, 8), solid=solid, inlet=np.zeros(8), outlet=True)
     for v in ("fused", "fused-f32s", "fused-fp16", "fused-bf16", "gather"):
         assert B.get("hip", v).supports(geo)[0], v
-    for v in ("fused-aa", "fused-aa-fp16", "fused-aa-bf16"):
+    for v in ("fused-aa", "fused-aa-fp16", "fused-aa-bf16", "fused-2step", "fused-2step-fp16",
+              "fused-2step-bf16"):
         ok, why = B.get("hip", v).supports(geo)
         assert not ok and "box" in why
     cls = B.get("hip", "fused")

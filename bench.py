@@ -20,7 +20,8 @@ power, temperatures and limiter counters recorded per leg.
 
 ``--profile [d2q9|d2q9-2step|d2q9-2step-fp16|d3q19|d3q19-fp16|d3q19-aa|d3q19-aa-fp16|geometry-d2q9|
 geometry-d3q19]`` runs only one step kernel (the headline by default) and the copy kernel (for
-rocprofv3);
+rocprofv3); ``--profile sweep`` runs the sweep-stream probe (the two-step kernel's access
+pattern);
 ``--profile copy`` runs every probe of the copy ceiling over 4 GiB buffers.
 """
 
@@ -274,8 +275,10 @@ def north_star_ratios(variants: list, adjoint: list, head: dict, copy_gbs: float
                       planes: dict | None = None) -> dict:
     """The north-star targets computed from this run's own legs: fp32 fraction of the copy
     ceiling, 16-bit over fp32 MLUPS per collision and pattern (legs of the same protocol),
-    forward+adjoint over forward-only, and each two-buffer leg's bandwidth as a fraction
-    of the plane stream of its own shape (``planes``, from plane_stream_bandwidth)."""
+    forward+adjoint over forward-only, each two-buffer leg's bandwidth as a fraction of the
+    plane stream of its own shape (``planes``, from plane_stream_bandwidth), and the
+    two-steps-per-launch legs over the one-step legs and as a fraction of the sweep stream
+    (``planes``, from sweep_stream_bandwidth)."""
     def leg(variant, coll, lattice="D2Q9"):
         for r in variants:
             if (r["variant"] == variant and r["collision"] == coll and r["lattice"] == lattice
@@ -317,6 +320,17 @@ def north_star_ratios(variants: list, adjoint: list, head: dict, copy_gbs: float
                 ("d3q19_bf16_trt", "d3q19_16", gbs("fused-bf16", "trt", "D3Q19"))):
             ref = planes.get(f"{shape}_GBs")
             out[f"{key}_frac_of_plane_stream"] = ratio(g, ref)
+        # two steps per launch: the state bytes of a launch over the sweep stream's rate
+        for st, shape in (("", "d2q9_sweep_f32"), ("-fp16", "d2q9_sweep_16"),
+                          ("-bf16", "d2q9_sweep_16")):
+            for coll in ("bgk", "trt", "mrt"):
+                key = f"d2q9_2step{st.replace('-', '_')}_{coll}_frac_of_sweep_stream"
+                out[key] = ratio(gbs(f"fused-2step{st}", coll, "D2Q9"),
+                                 planes.get(f"{shape}_GBs"))
+    for coll in ("bgk", "trt", "mrt"):
+        for st in ("", "-fp16", "-bf16"):
+            out[f"d2q9_2step{st.replace('-', '_')}_over_1step_{coll}"] = ratio(
+                leg(f"fused-2step{st}", coll), leg(f"fused{st}", coll))
     return out
 
 
@@ -336,7 +350,7 @@ def main() -> int:
     ap.add_argument("--profile", nargs="?", const="d2q9", default=None,
                     choices=["d2q9", "d2q9-2step", "d2q9-2step-fp16", "d3q19", "d3q19-fp16",
                              "d3q19-aa", "d3q19-aa-fp16",
-                             "geometry-d2q9", "geometry-d3q19", "copy", "planes"],
+                             "geometry-d2q9", "geometry-d3q19", "copy", "planes", "sweep"],
                     help="only one step kernel and the copy kernel, for rocprofv3: the headline "
                          "(d2q9, default), its two-steps-per-launch form (fp32 / fp16), D3Q19 "
                          "512^3 TRT fp32 / fp16, two-buffer or AA, or "
@@ -368,7 +382,8 @@ def main() -> int:
 
     from pylbm_amd import _lib
     from pylbm_amd.benchmark import (SPEC_BANDWIDTH_GBS, copy_bandwidth, device_name,
-                                     gpu_state, plane_stream_bandwidth)
+                                     gpu_state, plane_stream_bandwidth,
+                                     sweep_stream_bandwidth)
 
     if not torch.cuda.is_available():
         print("bench.py needs a HIP GPU", file=sys.stderr)
@@ -430,6 +445,17 @@ def main() -> int:
         from pylbm_amd import backends as B
         from pylbm_amd.problem import Problem
 
+        if a.profile == "sweep":
+            # the sweep stream (fp32, n x n) a.steps times, for its kernel trace and counters
+            nbytes = 9 * n * n * 4
+            x = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+            y = torch.empty_like(x)
+            st = int(torch.cuda.current_stream().cuda_stream)
+            for _ in range(a.steps):
+                _lib.call("pylbm_hip_sweep_probe", 4, x.data_ptr(), y.data_ptr(), n, n, st)
+            torch.cuda.synchronize()
+            log(f"profile: {a.steps} sweep streams of {n}^2 fp32 done")
+            return 0
         if a.profile.startswith("d2q9"):
             p = problem_2d("bgk", n, rho, u)
             v = {"d2q9": "fused", "d2q9-2step": "fused-2step",
@@ -483,6 +509,8 @@ def main() -> int:
         planes = plane_stream_bandwidth(
             {"d2q9_f32": (9, 4, n * n), "d2q9_16": (9, 2, n * n),
              "d3q19_f32": (19, 4, n3 ** 3), "d3q19_16": (19, 2, n3 ** 3)})
+        # the sweep stream: the ceiling of the two-steps-per-launch kernel's access pattern
+        planes.update(sweep_stream_bandwidth(n))
         log("plane streams: " + ", ".join(f"{k[:-4]} {v:.0f} GB/s" for k, v in planes.items()
                                           if k.endswith("_GBs") and "_max" not in k))
 

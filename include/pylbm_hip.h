@@ -258,6 +258,19 @@ int pylbm_hip_stream_probe(int kind, int unroll, const void* src, void* dst, int
 int pylbm_hip_plane_probe(int planes, int elem_bytes, const void* src, void* dst,
                           int64_t plane_bytes, void* stream);
 
+/*
+ * Sweep-stream probe: the access pattern of the two-steps-per-launch kernel (PYLBM_AB2)
+ * without its arithmetic.  src and dst each hold a periodic nx x ny D2Q9 state of
+ * elem_bytes (2 or 4) elements, 9 planes of nx ny, under 4 GiB; workgroups of 256 columns
+ * sweep chunks of rows as that kernel does, loading a row ahead, and store each row
+ * pulled from src: dst becomes one periodic streaming step of src,
+ * dst[i][y][x] = src[i][y - c_y,i][x - c_x,i].  Its rate (2 * 9 nx ny elem_bytes per
+ * launch) is the ceiling of that kernel's memory pattern.  Extends the copy probe of
+ * bench/roofline.py:30-51, which has one form only.
+ */
+int pylbm_hip_sweep_probe(int elem_bytes, const void* src, void* dst, int nx, int ny,
+                          void* stream);
+
 /* ---------------------------------------------------------------------------------------
  * Discrete adjoint of the two-buffer step (BGK / TRT, Guo forcing; params->aa must be 0).
  * No reference counterpart: the reference has no gradient path (its torch rungs run under

@@ -35,6 +35,7 @@ EXPORTS = (
     "pylbm_hip_links", "pylbm_hip_reduce_noalloc", "pylbm_hip_stream_probe",
     "pylbm_hip_plane_probe", "pylbm_hip_run_record", "pylbm_hip_adjoint_step_record",
     "pylbm_hip_adjoint_run_record", "pylbm_hip_link_groups", "pylbm_hip_run_pairs",
+    "pylbm_hip_sweep_probe",
 )
 
 
@@ -108,6 +109,7 @@ def load() -> Any:
         "pylbm_hip_copy": ([vp, vp, i64, vp], i32),
         "pylbm_hip_stream_probe": ([i32, i32, vp, vp, i64, vp], i32),
         "pylbm_hip_plane_probe": ([i32, i32, vp, vp, i64, vp], i32),
+        "pylbm_hip_sweep_probe": ([i32, vp, vp, i32, i32, vp], i32),
         "pylbm_hip_adjoint_workspace": ([pp], i64),
         "pylbm_hip_adjoint_seed": ([pp, vp, vp, vp, vp, vp], i32),
         "pylbm_hip_adjoint_step": ([pp, vp, vp, vp, vp, vp, vp], i32),

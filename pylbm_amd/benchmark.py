@@ -91,6 +91,34 @@ def plane_stream_bandwidth(shapes: dict | None = None, repeats: int = 20,
     return out
 
 
+def sweep_stream_bandwidth(n: int = 4096, repeats: int = 20, load_s: float = 0.1) -> dict:
+    """Rate of the sweep stream (``pylbm_hip_sweep_probe``) over a periodic n x n D2Q9 state
+    of fp32 and of 16-bit elements: the memory access pattern of the two-steps-per-launch
+    kernel without its arithmetic, counted as 2 x 9 n^2 element bytes per launch (the state
+    read and written once).  The ceiling for that kernel, beside the plane stream of the
+    one-step kernel (``plane_stream_bandwidth``)."""
+    torch = _torch()
+    out: dict[str, Any] = {}
+    stream = int(torch.cuda.current_stream().cuda_stream)
+    for name, eb in (("d2q9_sweep_f32", 4), ("d2q9_sweep_16", 2)):
+        nbytes = 9 * n * n * eb
+        a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+
+        def fn(a=a, b=b, eb=eb):
+            _lib.call("pylbm_hip_sweep_probe", eb, a.data_ptr(), b.data_ptr(), n, n, stream)
+
+        _load(fn, load_s)
+        times = _event_times(fn, repeats)
+        bw = [2 * nbytes / t / 1e9 for t in times]
+        out[f"{name}_GBs"] = statistics.median(bw)
+        out[f"{name}_max_GBs"] = max(bw)
+        out[f"{name}_us"] = statistics.median(times) * 1e6
+        del a, b
+        torch.cuda.empty_cache()
+    return out
+
+
 def copy_bandwidth(mib: int = 4096, repeats: int = 20, sweep: bool = True) -> dict:
     """Median device copy bandwidth (read + write counted, bench/roofline.py:30-51).
 

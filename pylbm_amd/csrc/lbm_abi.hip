@@ -16,6 +16,8 @@
 namespace plbm {
 int launch_step_pair_d2q9(const KArgs& a, int coll, int storage, int fmode, const void* src,
                           void* dst, hipStream_t s);
+int launch_sweep_probe_d2q9(const KArgs& a, int elem_bytes, const void* src, void* dst,
+                            hipStream_t s);
 template <class L>
 int launch_step(const KArgs& a, int coll, int storage, int pattern, int fmode,
                 const void* src, void* dst, hipStream_t s);
@@ -528,6 +530,24 @@ int pylbm_hip_plane_probe(int planes, int elem_bytes, const void* src, void* dst
   return hip_fail(plbm::launch_plane_probe(planes, elem_bytes, src, dst, plane_bytes,
                                            (hipStream_t)stream),
                   "pylbm_hip_plane_probe");
+}
+
+int pylbm_hip_sweep_probe(int elem_bytes, const void* src, void* dst, int nx, int ny,
+                          void* stream) {
+  if (elem_bytes != 2 && elem_bytes != 4)
+    return fail(PYLBM_EINVAL, "pylbm_hip_sweep_probe: elem_bytes must be 2 or 4");
+  if (!src || !dst || src == dst)
+    return fail(PYLBM_EINVAL, "pylbm_hip_sweep_probe: needs two distinct buffers");
+  if (nx < 1 || ny < 1) return fail(PYLBM_EINVAL, "pylbm_hip_sweep_probe: nx, ny must be >= 1");
+  if (9 * (int64_t)nx * ny * elem_bytes >= ((int64_t)1 << 32))
+    return fail(PYLBM_ERANGE, "pylbm_hip_sweep_probe: the state must span less than 4 GiB");
+  plbm::KArgs a{};
+  a.nx = nx;
+  a.ny = ny;
+  a.nz = 1;
+  a.N = nx * ny;
+  return hip_fail(plbm::launch_sweep_probe_d2q9(a, elem_bytes, src, dst, (hipStream_t)stream),
+                  "pylbm_hip_sweep_probe");
 }
 
 int64_t pylbm_hip_adjoint_workspace(const pylbm_params* p) {

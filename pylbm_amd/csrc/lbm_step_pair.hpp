@@ -222,6 +222,75 @@ int launch_step_pair_f(const KArgs& a, const void* src, void* dst, hipStream_t s
   return (int)hipGetLastError();
 }
 
+// Sweep-stream probe: the memory access pattern of k_step_pair without its arithmetic.  The
+// same tiles, chunks of rows (so the same grid: one wave of resident workgroups), XCD
+// order, loads one row ahead and ordinary stores; each row is pulled from src (slot i from
+// (x - c_x, y - c_y), periodic) and stored to dst, so dst becomes one streaming step of
+// src.  No halo rows: the rate counts the state bytes of a launch.
+template <int ST>
+__global__ void __launch_bounds__(kPairCols)
+k_sweep_probe(const KArgs a, const typename Codec<ST>::T* __restrict__ A,
+              typename Codec<ST>::T* __restrict__ B, int nxt, int Ly) {
+  using L = D2Q9;
+  using T = typename Codec<ST>::T;
+  constexpr int Q = L::Q;
+  constexpr uint32_t TS = sizeof(T);
+  const int nx = a.nx, ny = a.ny;
+  int u = (int)blockIdx.x;
+  const int full = (int)gridDim.x & ~7;
+  if (u < full) u = (u & 7) * (full >> 3) + (u >> 3);
+  const int xt = u % nxt, yc = u / nxt;
+  const int x0 = xt * kPairCols;
+  const int w = min(kPairCols, nx - x0);
+  const int y0 = yc * Ly, y1 = min(y0 + Ly, ny);
+  const int x = x0 + min((int)threadIdx.x, w - 1);
+  const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
+  const size_t plane = (size_t)a.N * TS;
+  const char* Ab = reinterpret_cast<const char*>(A);
+  char* Bb = reinterpret_cast<char*>(B);
+  auto wrapy = [&](int y) { return y < 0 ? y + ny : (y >= ny ? y - ny : y); };
+  T t[Q];
+  auto issue = [&](int p) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const int cx = L::c(i, 0);
+      const int xs = cx > 0 ? xm : (cx < 0 ? xp : x);
+      const ptrdiff_t row = (ptrdiff_t)nx * wrapy(p - L::c(i, 1)) * (ptrdiff_t)TS;
+      t[i] = ld_pl<T, 1>(Ab, plane, i, row, (uint32_t)xs * TS);
+    }
+  };
+  auto store = [&](int q, const T (&v)[Q]) {
+    const uint32_t off = (uint32_t)(x + nx * q) * TS;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) st_pl<0, 1>(Bb, plane, i, 0, off, v[i]);
+  };
+  issue(y0);
+  for (int r = y0; r + 1 < y1; ++r) {
+    T v[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) v[i] = t[i];
+    issue(r + 1);
+    __builtin_amdgcn_sched_barrier(0);
+    store(r, v);
+  }
+  store(y1 - 1, t);
+}
+
+// One sweep-stream launch over a periodic nx x ny D2Q9 state of storage ST (the caller checks
+// the arguments; state under 4 GiB), chunked as k_step_pair<BGK, ST> is.
+template <int ST>
+int launch_sweep_probe_t(const KArgs& a, const void* src, void* dst, hipStream_t s) {
+  using T = typename Codec<ST>::T;
+  const int nxt = (a.nx + kPairCols - 1) / kPairCols;
+  const int Ly = pair_chunk_rows(nxt, a.ny, pair_slots(k_step_pair<PYLBM_BGK, ST, kForceNone>));
+  const int nyc = (a.ny + Ly - 1) / Ly;
+  const int64_t wgs = (int64_t)nxt * nyc;
+  if (wgs > INT32_MAX) return PYLBM_ERANGE;
+  k_sweep_probe<ST><<<dim3((unsigned)wgs), dim3(kPairCols), 0, s>>>(a, (const T*)src, (T*)dst,
+                                                                     nxt, Ly);
+  return (int)hipGetLastError();
+}
+
 template <int COLL, int ST>
 int launch_step_pair_t(const KArgs& a, int fmode, const void* src, void* dst, hipStream_t s) {
   if (fmode == kForceField) return launch_step_pair_f<COLL, ST, kForceField>(a, src, dst, s);

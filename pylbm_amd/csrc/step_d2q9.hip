@@ -10,4 +10,10 @@ int launch_step_pair_d2q9(const KArgs& a, int coll, int storage, int fmode, cons
                           void* dst, hipStream_t s) {
   return launch_step_pair(a, coll, storage, fmode, src, dst, s);
 }
+// the access pattern of the kernel above without its arithmetic (pylbm_hip_sweep_probe)
+int launch_sweep_probe_d2q9(const KArgs& a, int elem_bytes, const void* src, void* dst,
+                            hipStream_t s) {
+  return elem_bytes == 4 ? launch_sweep_probe_t<PYLBM_F32>(a, src, dst, s)
+                         : launch_sweep_probe_t<PYLBM_F16>(a, src, dst, s);
+}
 }  // namespace plbm

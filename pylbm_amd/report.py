@@ -143,11 +143,17 @@ def _is_aa(run: dict) -> bool:
     return "-aa" in run["variant"] or run["variant"].endswith("aa")
 
 
+def _is_pair(run: dict) -> bool:
+    """Two steps per launch (the hip ``fused-2step*`` rungs)."""
+    return "2step" in run["variant"]
+
+
 def headline(bench: dict) -> list[str]:
     """The metric's own number when the record has one, then the best D2Q9 rates: the
     two-buffer pull kernels (fp32, 16-bit and their ratio) and, beside them, the in-place AA
     kernels, whose alternating sweeps let the 256 MiB Infinity Cache serve part of each
-    launch's reads (so they can pass the streaming copy); then the best fp32 D3Q19 rate.
+    launch's reads (so they can pass the streaming copy), and the two-steps-per-launch
+    kernels against the one-step pull; then the best fp32 D3Q19 rate.
     The reference's float64 NumPy rung is not a candidate."""
     gpu = bench.get("machine", {}).get("gpu", "GPU")
     bw = bench.get("ceiling", {}).get("copy_bandwidth_GBs")
@@ -163,8 +169,9 @@ def headline(bench: dict) -> list[str]:
                       if h.get("pct_copy") is not None else "")
                    + (f", {100 * frac:.1f}% of the 8 TB/s spec" if frac else "") + ".")
     d2 = [r for r in ladder(bench, "D2Q9") if r["backend"] != "numpy"]
-    pull = [r for r in d2 if not _is_aa(r)]
+    pull = [r for r in d2 if not _is_aa(r) and not _is_pair(r)]
     aa = [r for r in d2 if _is_aa(r)]
+    pair = [r for r in d2 if _is_pair(r)]
 
     def best(rs: list[dict], sixteen: bool) -> dict | None:
         c = [r for r in rs if _is16(r) == sixteen]
@@ -194,6 +201,18 @@ def headline(bench: dict) -> list[str]:
         if a32 and a16:
             line += f"; 16-bit / fp32 {a16['median'] / a32['median']:.2f}x"
         out.append(line + ".")
+    t32, t16 = best(pair, False), best(pair, True)
+    if t32 or t16:
+        parts = []
+        for r, one, name in ((t32, p32, "fp32"), (t16, p16, "16-bit")):
+            if r:
+                part = (f"{name} **{r['median']:,.0f} MLUPS** ({r['backend']}/{r['variant']}, "
+                        f"{r['bytes_per_unit']} B/update")
+                if one:
+                    part += f", {r['median'] / one['median']:.2f}x the {name} pull rung"
+                parts.append(part + ")")
+        out.append("Two steps per launch (periodic boxes; the state crosses memory once per "
+                   "two steps, the intermediate one stays in LDS): " + ", ".join(parts) + ".")
     d3 = [r for r in ladder(bench, "D3Q19") + ladder(bench, "D3Q19", "trt")
           if r["backend"] != "numpy" and not _is16(r)]
     if d3:
@@ -231,6 +250,12 @@ def performance_table(bench: dict, collisions: Iterable[str] = ("bgk", "trt", "m
     if got:
         note += (" Plane stream (the two-buffer kernels' access pattern without arithmetic, "
                  "pylbm_hip_plane_probe) at the legs' state shapes: " + ", ".join(got)
+                 + " GB/s.")
+    sw = [f"{label} {ps[k + '_GBs']:,.0f}" for k, label in
+          (("d2q9_sweep_f32", "fp32"), ("d2q9_sweep_16", "16-bit")) if ps.get(k + "_GBs")]
+    if sw:
+        note += (" Sweep stream (the two-steps-per-launch kernels' access pattern without "
+                 "arithmetic, pylbm_hip_sweep_probe) at D2Q9 4096^2: " + ", ".join(sw)
                  + " GB/s.")
     return "\n".join(rows) + ("\n\n" + note if note else "")
 

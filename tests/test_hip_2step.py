@@ -182,3 +182,13 @@ def test_2step_at_4096_equals_single_steps(storage):
     print(f"4096^2 {storage}: largest difference {d} ulp; mass {m0:.10e} -> {m1:.10e}")
     assert d == 0
     assert abs(m1 - m0) < (1e-9 if storage == "f32" else 1e-6) * m0
+
+
+def test_abi_rejects_bad_sweep_probe_calls():
+    lib = _lib.load()
+    a, b = ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000)
+    assert lib.pylbm_hip_sweep_probe(3, a, b, 64, 64, None) == 1001
+    assert lib.pylbm_hip_sweep_probe(4, a, a, 64, 64, None) == 1001
+    assert lib.pylbm_hip_sweep_probe(4, a, b, 0, 64, None) == 1001
+    assert lib.pylbm_hip_sweep_probe(4, a, b, 16384, 16384, None) == 1002
+    assert "4 GiB" in lib.pylbm_hip_last_error().decode()

@@ -1,7 +1,9 @@
 """The bandwidth probes move exactly the bytes they claim (GPU).
 
-The copy ceiling (pylbm_hip_copy, pylbm_hip_stream_probe) and the plane-stream ceiling
-(pylbm_hip_plane_probe) are only ceilings if every probe touches every byte: each copy
+The copy ceiling (pylbm_hip_copy, pylbm_hip_stream_probe), the plane-stream ceiling
+(pylbm_hip_plane_probe) and the sweep-stream ceiling (pylbm_hip_sweep_probe) are only
+ceilings if every probe touches every byte (the sweep probe's destination must be one
+streaming step of its source): each copy
 form must leave dst equal to src, byte for byte, on a destination filled with NaN first;
 the write-only stream must fill dst; the read-only stream must leave a 1.0f-filled dst
 untouched beyond its one word per workgroup.  Sizes exercise partial last workgroups
@@ -85,3 +87,28 @@ def test_plane_probe_rejects_bad_shapes(args):
     with pytest.raises(_lib.HipError):
         _lib.call("pylbm_hip_plane_probe", planes, eb, buf.data_ptr(), buf.data_ptr(), plane,
                   _stream())
+
+
+# D2Q9 velocities in the library's order (rest, E, N, W, S, NE, NW, SW, SE)
+_C9 = [(0, 0), (1, 0), (0, 1), (-1, 0), (0, -1), (1, 1), (-1, 1), (-1, -1), (1, -1)]
+
+
+@pytest.mark.parametrize("elem_bytes", [4, 2])
+@pytest.mark.parametrize("shape", [(600, 71), (256, 10), (1, 3), (4096, 37)])
+def test_sweep_probe_is_one_streaming_step(elem_bytes, shape):
+    """The sweep-stream probe reads and writes every element it is timed for: dst is one
+    periodic pull-streaming step of src, dst[i][y][x] = src[i][y - c_y][x - c_x], bit for
+    bit, on a NaN-filled destination (partial last tiles, chunks of rows with a shorter
+    last one, a one-column state)."""
+    nx, ny = shape
+    n = nx * ny
+    dt = torch.int32 if elem_bytes == 4 else torch.int16
+    g = torch.Generator(device="cpu").manual_seed(nx * 1000 + ny)
+    src = torch.randint(-2**15, 2**15 - 1, (9, ny, nx), generator=g).to(dt).cuda()
+    dst = torch.full_like(src, -1)
+    _lib.call("pylbm_hip_sweep_probe", elem_bytes, src.data_ptr(), dst.data_ptr(), nx, ny,
+              _stream())
+    torch.cuda.synchronize()
+    want = torch.stack([torch.roll(src[i], shifts=(cy, cx), dims=(0, 1))
+                        for i, (cx, cy) in enumerate(_C9)])
+    assert torch.equal(dst, want), f"{shape} {elem_bytes} B, {n} nodes"

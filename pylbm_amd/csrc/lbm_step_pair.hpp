@@ -22,6 +22,8 @@
 // collision on the same encoded populations.
 #pragma once
 
+#include <atomic>
+
 #include "lbm_step.hpp"
 
 namespace plbm {
@@ -192,20 +194,22 @@ inline int pair_chunk_rows(int nxt, int ny, int slots) {
   return Ly | 1;
 }
 
-// workgroup slots of kernel k on the current device (cached per device)
+// workgroup slots of kernel k on the current device (cached per device; host threads may
+// race to fill an entry, with the same value)
 template <class K>
 int pair_slots(K k) {
-  static int cache[64] = {};
+  static std::atomic<int> cache[64] = {};
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 2048;
-  if (cache[dev] > 0) return cache[dev];
+  const int known = cache[dev].load(std::memory_order_relaxed);
+  if (known > 0) return known;
   int cus = 0, nb = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
       hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, kPairCols, 0) != hipSuccess ||
       cus <= 0 || nb <= 0)
     return 2048;
-  cache[dev] = cus * nb;
-  return cache[dev];
+  cache[dev].store(cus * nb, std::memory_order_relaxed);
+  return cus * nb;
 }
 
 template <int COLL, int ST, int FM>

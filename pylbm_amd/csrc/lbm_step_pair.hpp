@@ -15,8 +15,8 @@
 //     which lanes 0 and 1 of the first wave compute;
 //   * step A also runs on the halo rows y0 - 1 and y1 (1 + 2 / Ly of the reads);
 //   * step A's loads of row r are issued before step B of row r - 2 runs, so that they fly
-//     while it computes; the y-chunks are sized so that the grid is one full wave of
-//     resident workgroups, and each XCD takes a contiguous range of tiles.
+//     while it computes; the y-chunks are short (pair_chunk_rows), and each XCD takes a
+//     contiguous range of tiles.
 // Per node and two steps: src read once (plus the halo), dst written once, half the bytes of
 // two single steps.  Every value is the single steps' value: the same gather, decode and
 // collision on the same encoded populations.
@@ -181,17 +181,24 @@ k_step_pair(const KArgs a, const typename Codec<ST>::T* __restrict__ A,
   stepB(y1 - 1);
 }
 
-// Rows per y-chunk: one chunk per workgroup slot of the GPU (CUs x resident workgroups of
-// the kernel), so that the grid runs as one full wave of workgroups, with at least 8 rows
-// per chunk (few halo rows); rows wider than the GPU's slots take chunks of 65 rows.  The
-// height is odd: chunks a power of two of rows apart, swept in step, meet the same memory
-// channels at once (measured: a quarter of the bandwidth lost at 32 rows).
+// Rows per y-chunk: the height that makes the grid one wave of resident workgroups (CUs x
+// workgroups per CU), capped at kPairRowsMax, odd from 3 rows up.  Measured on MI355X (BGK,
+// fp32 / fp16, us per step against the height):
+//   * small states (512^2: 1 row, 3.7 us against 7.9 at 9; 1024^2: 2 rows, 8.5 / 7.7 us
+//     against 9.7 / 10.2 at 9): the sweep's row latency sets the time, so short chunks win
+//     although step A then repeats 2 halo rows per chunk (the state sits in the caches);
+//   * large states (8192^2: 9 rows, 497 / 281 us against 551 / 306 at one wave of 129-row
+//     chunks; 4096^2 within 2% from 5 to 65 rows): many waves of short chunks keep the rows
+//     in flight close together, as a one-pass kernel does, which outweighs the 2/9 halo;
+//   * chunks a power of two of rows apart, swept in step, meet the same memory channels at
+//     once (a quarter of the bandwidth lost at 32 rows with one wave): hence odd heights.
+constexpr int kPairRowsMax = 9;
+
 inline int pair_chunk_rows(int nxt, int ny, int slots) {
-  const int per = slots / nxt;
-  if (per < 1) return 65;
-  int Ly = (ny + per - 1) / per;
-  Ly = Ly < 9 ? 9 : Ly;
-  return Ly | 1;
+  int Ly = (int)(((int64_t)ny * nxt + slots - 1) / slots);
+  if (Ly >= kPairRowsMax) return kPairRowsMax;
+  if (Ly >= 3) Ly |= 1;
+  return Ly < 1 ? 1 : Ly;
 }
 
 // workgroup slots of kernel k on the current device (cached per device; host threads may
@@ -227,7 +234,7 @@ int launch_step_pair_f(const KArgs& a, const void* src, void* dst, hipStream_t s
 }
 
 // Sweep-stream probe: the memory access pattern of k_step_pair without its arithmetic.  The
-// same tiles, chunks of rows (so the same grid: one wave of resident workgroups), XCD
+// same tiles, chunks of rows (so the same grid), XCD
 // order, loads one row ahead and ordinary stores; each row is pulled from src (slot i from
 // (x - c_x, y - c_y), periodic) and stored to dst, so dst becomes one streaming step of
 // src.  No halo rows: the rate counts the state bytes of a launch.

@@ -66,6 +66,11 @@ CASES = {
     "40x7-trt": dict(shape=(40, 7), tau=0.7, collision="trt"),
     "256x3-bgk": dict(shape=(256, 3), tau=0.8, collision="bgk", force=(2e-5, 1e-5)),
     "1x5-mrt": dict(shape=(1, 5), tau=0.7, collision="mrt"),
+    # one row, one node, two rows, rows of 21 tiles (the last partial) in chunks of one row
+    "300x1-trt": dict(shape=(300, 1), tau=0.66, collision="trt"),
+    "1x1-bgk": dict(shape=(1, 1), tau=0.9, collision="bgk", force=(1e-5, 2e-5)),
+    "257x2-mrt": dict(shape=(257, 2), tau=0.6, collision="mrt"),
+    "5300x6-bgk": dict(shape=(5300, 6), tau=0.7, collision="bgk"),
     # a per-cell force field
     "300x40-bgk-field": dict(shape=(300, 40), tau=0.6, collision="bgk", field=True),
     "300x40-mrt-field": dict(shape=(300, 40), tau=0.65, collision="mrt", field=True),

@@ -96,9 +96,9 @@ const char* pylbm_hip_last_error(void);
  * Guo forcing, BGK/TRT/MRT collision, store.  pattern = PYLBM_AB reads src writes dst
  * (src != dst); PYLBM_AA_ODD / PYLBM_AA_EVEN work in place (src == dst required).
  * PYLBM_AB2 advances two steps in one launch, src -> dst, with the intermediate state on
- * chip (the same values as two PYLBM_AB steps); D2Q9 boxes periodic on both axes whose
- * state is under 4 GiB only (PYLBM_ENOTSUP otherwise).  No reference counterpart for
- * PYLBM_AB2.
+ * chip (the same values as two PYLBM_AB steps); D2Q9 box problems (periodic axes or walls,
+ * the lid; no links, inlet or outlet) whose state is under 4 GiB only (PYLBM_ENOTSUP
+ * otherwise).  No reference counterpart for PYLBM_AB2.
  * Replaces FusedSolver._launch (pylbm/backends/_fused.py:108-110,
  * pylbm/backends/cupy_backend.py:101-103) and the kernel lbm_kernel
  * (pylbm/backends/cuda/lbm.cu:50-53).
@@ -117,8 +117,8 @@ int pylbm_hip_run(const pylbm_params* p, void* a, void* b, int64_t nsteps, void*
 
 /*
  * nsteps (even) time steps as nsteps / 2 launches of the PYLBM_AB2 pattern, ping-ponging
- * a -> b -> a ...; after an odd nsteps / 2 the state is in b.  D2Q9 boxes periodic on both
- * axes with a state under 4 GiB (PYLBM_ENOTSUP otherwise).  The step loop of the
+ * a -> b -> a ...; after an odd nsteps / 2 the state is in b.  D2Q9 box problems with a
+ * state under 4 GiB (PYLBM_ENOTSUP otherwise).  The step loop of the
  * two-steps-per-launch variants; no reference counterpart (a form of
  * FusedSolver._advance, pylbm/backends/_fused.py:116-124).
  */

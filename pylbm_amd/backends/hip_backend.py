@@ -66,7 +66,7 @@ class HipFused(Solver):
     description = "hand-written HIP (gfx950), fused pull, two buffers, fp32"
     storage: ClassVar[str] = "f32"
     aa: ClassVar[bool] = False
-    #: two time steps per launch (the intermediate state on chip; periodic D2Q9 boxes)
+    #: two time steps per launch (the intermediate state on chip; D2Q9 boxes)
     pair: ClassVar[bool] = False
     lattices: ClassVar[tuple[str, ...]] = ("D2Q9", "D3Q19")
     collisions: ClassVar[tuple[str, ...]] = ("bgk", "trt", "mrt")
@@ -86,8 +86,8 @@ class HipFused(Solver):
                 return False, "with obstacles, inlet or outlet the state must be < 4 GiB"
         if problem.lattice.name not in cls.lattices:
             return False, f"{problem.lattice.name} not implemented"
-        if cls.pair and not (problem.is_box() and all(problem.periodic)):
-            return False, "two steps per launch run on boxes periodic on every axis"
+        if cls.pair and not problem.is_box():
+            return False, "two steps per launch run on box problems (no obstacles, inlet or outlet)"
         if cls.pair and problem.lattice.q * problem.n_nodes * item >= 2**32:
             return False, "two steps per launch need the state under 4 GiB"
         if problem.collision not in cls.collisions:
@@ -264,7 +264,7 @@ class HipFused(Solver):
     # ------------------------------------------------------------------ stepping
     def launch(self, pattern: str) -> None:
         """One kernel launch: 'ab' (A -> B, then swap), 'ab2' (two steps A -> B, then swap;
-        periodic D2Q9 boxes), 'aa-odd' or 'aa-even' (in place)."""
+        D2Q9 boxes), 'aa-odd' or 'aa-even' (in place)."""
         pid = _lib.PATTERN_IDS[pattern]
         if pid in (_lib.AB, _lib.AB2):
             _lib.call("pylbm_hip_step", self.pref, _ptr(self.A), _ptr(self.B), pid, _stream())
@@ -340,9 +340,9 @@ class HipFusedAABF16(HipFused):
 
 
 class HipFused2Step(HipFused):
-    """Two time steps per launch on D2Q9 boxes periodic on both axes: the first step's
-    populations stay in LDS, so the state crosses memory once per two steps (no reference
-    counterpart; the values are those of two ``fused`` steps)."""
+    """Two time steps per launch on D2Q9 boxes (periodic axes or bounce-back walls, the
+    lid): the first step's populations stay in LDS, so the state crosses memory once per
+    two steps (no reference counterpart; the values are those of two ``fused`` steps)."""
 
     variant = "fused-2step"
     description = ("hand-written HIP (gfx950), fused pull, two steps per launch, state between "

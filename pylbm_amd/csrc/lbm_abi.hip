@@ -181,12 +181,13 @@ int make_kargs(const pylbm_params* p, KArgs& a, const char* what) {
   return 0;
 }
 
-// the two-steps-per-launch pattern: D2Q9 boxes periodic on both axes, the state under 4 GiB
-// (the kernel addresses it through one buffer descriptor)
+// the two-steps-per-launch pattern: D2Q9 box problems (periodic axes or bounce-back walls,
+// the lid; no obstacles, inlet or outlet), the state under 4 GiB (the kernel addresses it
+// through one buffer descriptor)
 bool pair_steps_apply(const pylbm_params* p) {
   const uint64_t ts = (p->storage == PYLBM_F16 || p->storage == PYLBM_BF16) ? 2 : 4;
-  return p->lattice == PYLBM_D2Q9 && p->periodic[0] && p->periodic[1] && !p->links &&
-         !p->inlet && !p->outlet && !p->aa && 9ull * (uint64_t)p->n_nodes * ts < (1ull << 32);
+  return p->lattice == PYLBM_D2Q9 && !p->links && !p->inlet && !p->outlet && !p->aa &&
+         9ull * (uint64_t)p->n_nodes * ts < (1ull << 32);
 }
 
 int step_impl(const pylbm_params* p, const KArgs& a, const void* src, void* dst, int pattern,
@@ -281,8 +282,8 @@ int pylbm_hip_step(const pylbm_params* p, const void* src, void* dst, int patter
     if (src == dst) return fail(PYLBM_EINVAL, "pylbm_hip_step: AB needs src != dst");
     if (p->aa) return fail(PYLBM_EINVAL, "pylbm_hip_step: AB pattern on an AA-layout state");
     if (pattern == PYLBM_AB2 && !pair_steps_apply(p))
-      return fail(PYLBM_ENOTSUP, "pylbm_hip_step: two steps per launch run on D2Q9 boxes "
-                  "periodic on both axes with a state under 4 GiB only");
+      return fail(PYLBM_ENOTSUP, "pylbm_hip_step: two steps per launch run on D2Q9 box "
+                  "problems with a state under 4 GiB only");
   } else if (pattern == PYLBM_AA_ODD || pattern == PYLBM_AA_EVEN) {
     if (src != dst) return fail(PYLBM_EINVAL, "pylbm_hip_step: AA patterns work in place");
   } else {
@@ -341,8 +342,8 @@ int pylbm_hip_run_pairs(const pylbm_params* p, void* a_buf, void* b_buf, int64_t
   if (!a_buf || !b_buf || a_buf == b_buf)
     return fail(PYLBM_EINVAL, "pylbm_hip_run_pairs: needs two distinct buffers");
   if (!pair_steps_apply(p))
-    return fail(PYLBM_ENOTSUP, "pylbm_hip_run_pairs: D2Q9 boxes periodic on both axes with a "
-                "state under 4 GiB only");
+    return fail(PYLBM_ENOTSUP, "pylbm_hip_run_pairs: D2Q9 box problems with a state under "
+                "4 GiB only");
   KArgs a;
   if (int e = make_kargs(p, a, "pylbm_hip_run_pairs")) return e;
   void* src = a_buf;

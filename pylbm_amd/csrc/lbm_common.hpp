@@ -209,6 +209,30 @@ __host__ __device__ constexpr int pair_lo(int k) {
   }
   return 0;
 }
+// c_i . v with the zero components of c_i left out and the unit ones as plain adds and
+// subtractions (the components of every lattice here are -1, 0 or 1).  The sum of the kept
+// terms is the value of the full product c_i,x v_x + c_i,y v_y + c_i,z v_z, which adds only
+// signed zeros to it, up to the sign of a zero result, and saves the vector instructions
+// that multiply by zero (they cannot be folded away: 0 * x is not 0 for every float x).
+template <class L>
+__device__ __forceinline__ float cdot(int i, float x, float y, float z) {
+  float s = 0.0f;
+  bool first = true;
+  auto term = [&](int c, float v) {
+    if (c == 0) return;
+    if (first) {
+      s = c > 0 ? v : -v;
+      first = false;
+    } else {
+      s = c > 0 ? s + v : s - v;
+    }
+  };
+  term(L::c(i, 0), x);
+  term(L::c(i, 1), y);
+  if (L::D == 3) term(L::c(i, 2), z);
+  return s;
+}
+
 // Per-node collision in fp32, written on deviations from the rest weights:
 // h_i = f_i - w_i in, p_i - w_i out.  With rho = 1 + sum h, u = (sum c h + F/2) / rho and
 // feq_i - w_i = w_i (drho + rho (3 cu + 4.5 cu^2 - 1.5 u^2)), every operator below is the
@@ -251,11 +275,11 @@ __device__ __forceinline__ void collide_bgk(const float (&h)[L::Q], float (&d)[L
   const float g = 1.0f - 0.5f * r0;
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
-    const float cu = L::c(i, 0) * ux + L::c(i, 1) * uy + L::c(i, 2) * uz;
+    const float cu = cdot<L>(i, ux, uy, uz);
     const float w = (float)L::w(i);
     const float poly = cu * (3.0f + 4.5f * cu) - 1.5f * usq;
     const float neq = h[i] - w * (drho + rho * poly);  // f - feq = h - (feq - w)
-    const float cf = L::c(i, 0) * F[0] + L::c(i, 1) * F[1] + L::c(i, 2) * F[2];
+    const float cf = cdot<L>(i, F[0], F[1], F[2]);
     const float S = FORCE ? w * (3.0f * (cf - uf) + 9.0f * cu * cf) : 0.0f;
     d[i] = FORCE ? h[i] - r0 * neq + g * S : h[i] - r0 * neq;
   }
@@ -311,12 +335,12 @@ __device__ __forceinline__ void collide_pairs(const float (&h)[L::Q], float (&d)
 #pragma unroll
     for (int k = 0; k < NP; ++k) {
       const int i = pair_lo<L>(k);
-      const float cu = L::c(i, 0) * ux + L::c(i, 1) * uy + L::c(i, 2) * uz;
+      const float cu = cdot<L>(i, ux, uy, uz);
       const float w = (float)L::w(i);
       sn[k] = hs[k] - w * (drho2 + rho * (9.0f * cu * cu - usq3));
       dn[k] = hd[k] - (6.0f * w) * rho * cu;
       if (FORCE) {
-        const float cf = L::c(i, 0) * F[0] + L::c(i, 1) * F[1] + L::c(i, 2) * F[2];
+        const float cf = cdot<L>(i, F[0], F[1], F[2]);
         ss[k] = w * (18.0f * cu * cf - 6.0f * uf);
         ds[k] = (6.0f * w) * cf;
       } else {

@@ -5,7 +5,7 @@
 namespace plbm {
 template int launch_step<D2Q9>(const KArgs&, int, int, int, int, const void*, void*,
                                hipStream_t);
-// two steps per launch on periodic boxes (lbm_step_pair.hpp)
+// two steps per launch on box problems (lbm_step_pair.hpp)
 int launch_step_pair_d2q9(const KArgs& a, int coll, int storage, int fmode, const void* src,
                           void* dst, hipStream_t s) {
   return launch_step_pair(a, coll, storage, fmode, src, dst, s);

@@ -211,7 +211,7 @@ def headline(bench: dict) -> list[str]:
                 if one:
                     part += f", {r['median'] / one['median']:.2f}x the {name} pull rung"
                 parts.append(part + ")")
-        out.append("Two steps per launch (periodic boxes; the state crosses memory once per "
+        out.append("Two steps per launch (the periodic box; the state crosses memory once per "
                    "two steps, the intermediate one stays in LDS): " + ", ".join(parts) + ".")
     d3 = [r for r in ladder(bench, "D3Q19") + ladder(bench, "D3Q19", "trt")
           if r["backend"] != "numpy" and not _is16(r)]

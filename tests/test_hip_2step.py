@@ -1,4 +1,5 @@
-"""Two time steps per launch (the fused-2step variants, PYLBM_AB2, pylbm_hip_run_pairs).
+"""Two time steps per launch (the fused-2step variants, PYLBM_AB2, pylbm_hip_run_pairs),
+on periodic boxes and with walls (half-way bounce-back, the lid).
 
 The kernel keeps the first step's populations in LDS, encoded exactly as a stored state, so
 its result is the result of two single two-buffer steps: the same gather, decode and
@@ -26,16 +27,18 @@ VARIANTS = {"f32": ("fused", "fused-2step"), "f16": ("fused-fp16", "fused-2step-
 # ----------------------------------------------------------------------------- CPU
 
 
-def test_2step_supports_periodic_d2q9_boxes_only():
+def test_2step_supports_d2q9_boxes_only():
     cls = B.get("hip", "fused-2step")
     assert cls.supports(Problem((64, 32), periodic=(True, True)))[0]
-    ok, why = cls.supports(Problem((64, 32), periodic=(True, False)))
-    assert not ok and "periodic" in why
+    assert cls.supports(Problem((64, 32), periodic=(True, False)))[0]
+    assert cls.supports(Problem((64, 32), u_lid=0.1))[0]
     ok, why = cls.supports(Problem((8, 8, 8), periodic=(True, True, True)))
     assert not ok and "D3Q19" in why
     ok, why = cls.supports(Problem((64, 32), periodic=(True, True),
                                    solid=np.zeros((32, 64), bool)))
-    assert not ok
+    assert not ok and "box" in why
+    ok, why = cls.supports(Problem((64, 32), inlet=np.zeros(32), outlet=True))
+    assert not ok and "box" in why
 
 
 def test_abi_rejects_two_steps_where_they_do_not_apply():
@@ -46,10 +49,17 @@ def test_abi_rejects_two_steps_where_they_do_not_apply():
     p.nx, p.ny, p.nz, p.n_nodes = 16, 8, 1, 128
     p.rates[0] = 1.0
     a, b = ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000)
+    # obstacles (a link mask) are not a box problem
+    p.links = ctypes.c_void_p(0x3000)
     assert lib.pylbm_hip_step(ctypes.byref(p), a, b, _lib.AB2, None) == 1003
-    assert "periodic" in lib.pylbm_hip_last_error().decode()
+    assert "box" in lib.pylbm_hip_last_error().decode()
     assert lib.pylbm_hip_run_pairs(ctypes.byref(p), a, b, 4, None) == 1003
-    p.periodic[1] = 1
+    p.links = None
+    # a D2Q9 state of 4 GiB or more
+    p.nx, p.ny, p.n_nodes = 16384, 8192, 16384 * 8192
+    assert lib.pylbm_hip_run_pairs(ctypes.byref(p), a, b, 4, None) == 1003
+    assert "4 GiB" in lib.pylbm_hip_last_error().decode()
+    p.nx, p.ny, p.n_nodes = 16, 8, 128
     assert lib.pylbm_hip_run_pairs(ctypes.byref(p), a, b, 3, None) == 1001
     assert lib.pylbm_hip_run_pairs(ctypes.byref(p), a, a, 4, None) == 1001
     assert lib.pylbm_hip_step(ctypes.byref(p), a, a, _lib.AB2, None) == 1001
@@ -71,6 +81,20 @@ CASES = {
     "1x1-bgk": dict(shape=(1, 1), tau=0.9, collision="bgk", force=(1e-5, 2e-5)),
     "257x2-mrt": dict(shape=(257, 2), tau=0.6, collision="mrt"),
     "5300x6-bgk": dict(shape=(5300, 6), tau=0.7, collision="bgk"),
+    # walls: the lid-driven cavity (corners, a partial tile), a channel (walls in y only),
+    # walls in x only, one column, two rows
+    "120x70-cavity-bgk": dict(shape=(120, 70), tau=0.6, collision="bgk", u_lid=0.05,
+                              periodic=(False, False)),
+    "600x40-cavity-trt-force": dict(shape=(600, 40), tau=0.7, collision="trt", u_lid=0.05,
+                                    force=(1e-5, 2e-6), periodic=(False, False)),
+    "300x50-channel-mrt": dict(shape=(300, 50), tau=0.62, collision="mrt", force=(2e-5, 0.0),
+                               periodic=(True, False)),
+    "64x33-xwalls-bgk-field": dict(shape=(64, 33), tau=0.65, collision="bgk", field=True,
+                                   periodic=(False, True)),
+    "1x9-walls-trt": dict(shape=(1, 9), tau=0.8, collision="trt", u_lid=0.1,
+                          periodic=(False, False)),
+    "257x2-cavity-mrt": dict(shape=(257, 2), tau=0.7, collision="mrt", u_lid=0.03,
+                             periodic=(False, False)),
     # a per-cell force field
     "300x40-bgk-field": dict(shape=(300, 40), tau=0.6, collision="bgk", field=True),
     "300x40-mrt-field": dict(shape=(300, 40), tau=0.65, collision="mrt", field=True),
@@ -79,10 +103,11 @@ CASES = {
 
 def _problem(kw: dict) -> Problem:
     kw = dict(kw)
+    kw.setdefault("periodic", (True, True))
     if kw.pop("field", False):
         nx, ny = kw["shape"]
         kw["force_field"] = 1e-5 * np.random.default_rng(5).standard_normal((2, ny, nx))
-    return Problem(**kw, periodic=(True, True))
+    return Problem(**kw)
 
 
 def _state(p, rng):

@@ -56,6 +56,8 @@ PAIR_CASES = {
                             force=(1e-5, 0.0)),
     # two chunks of rows, the last one row high
     "2d-one-row-chunk": dict(shape=(300, 10), tau=0.7, collision="trt", periodic=(True, True)),
+    # walls on both axes and the lid
+    "2d-walls-lid": SMALL["2d-walls-lid"],
 }
 
 

@@ -9,8 +9,8 @@ seeded generator.  Populations are resident in HBM before timing; after a load p
 ``--warm-time`` seconds and ``warmup`` untimed steps, exactly ``steps`` launches are timed
 with HIP events on the launch stream (median of three such windows).  The same line carries
 every other variant of that configuration (TRT, MRT, AA, fp16/bf16 shifted storage, two
-steps per launch), the
-D3Q19 512^3 TRT Taylor-Green runs (fp32, fp16, bf16; pull and AA), the geometry kernels,
+steps per launch), one step against two per launch on the walled 4096^2 cavity and channel,
+the D3Q19 512^3 TRT Taylor-Green runs (fp32, fp16, bf16; pull and AA), the geometry kernels,
 the forward+adjoint rollouts, the measured copy ceiling, the roofline of the dominant
 kernel, the north-star ratios of this run and the float64 NumPy oracle timed on the host
 (CPU baseline).  Every leg other than the headline follows the reference's protocol
@@ -18,11 +18,12 @@ kernel, the north-star ratios of this run and the float64 NumPy oracle timed on 
 step count, each at least ``--min-time`` seconds, median reported, with the GPU's clocks,
 power, temperatures and limiter counters recorded per leg.
 
-``--profile [d2q9|d2q9-2step|d2q9-2step-fp16|d3q19|d3q19-fp16|d3q19-aa|d3q19-aa-fp16|geometry-d2q9|
-geometry-d3q19]`` runs only one step kernel (the headline by default) and the copy kernel (for
-rocprofv3); ``--profile sweep`` runs the sweep-stream probe (the two-step kernel's access
-pattern);
-``--profile copy`` runs every probe of the copy ceiling over 4 GiB buffers.
+``--profile [d2q9|d2q9-2step|d2q9-2step-fp16|cavity|cavity-fp16|cavity-2step|cavity-2step-fp16|
+d3q19|d3q19-fp16|d3q19-aa|d3q19-aa-fp16|geometry-d2q9|geometry-d3q19]`` runs only one step
+kernel (the headline by default) and the copy kernel (for rocprofv3); the ``cavity`` forms run
+the lid-driven cavity at the headline size. ``--profile sweep`` runs the sweep-stream probe
+(the two-step kernel's access pattern); ``--profile copy`` runs every probe of the copy
+ceiling over 4 GiB buffers.
 """
 
 from __future__ import annotations
@@ -171,6 +172,27 @@ def geometry_legs(n2: int, proto: dict) -> list:
         out.append(r)
         log(f"  geometry D3Q19 {n3}^3 trt hip/{v}: {r['MLUPS']:.0f} MLUPS {r['GBs']:.0f} GB/s")
     torch.cuda.empty_cache()
+    return out
+
+
+def walled_legs(n2: int, proto: dict) -> list:
+    """One step against two steps per launch on walled D2Q9 boxes at the headline size: the
+    lid-driven cavity (bounce-back on all four walls, Ladd's lid at u = 0.1) and a channel
+    (periodic in x, bounce-back walls in y, uniform force 1e-6 in +x), BGK tau 0.6, fp32
+    and fp16 storage, from rest."""
+    from pylbm_amd.problem import Problem
+
+    out = []
+    cases = (("cavity", Problem((n2, n2), tau=0.6, collision="bgk", u_lid=0.1)),
+             ("channel", Problem((n2, n2), tau=0.6, collision="bgk", periodic=(True, False),
+                                 force=(1e-6, 0.0))))
+    for name, p in cases:
+        for v in ("fused", "fused-2step", "fused-fp16", "fused-2step-fp16"):
+            r = run_variant(v, p, (None, None), proto)
+            r["workload"] = f"walled {name} {n2}^2"
+            out.append(r)
+            log(f"  walled {name} {n2}^2 bgk hip/{v}: {r['MLUPS']:.0f} MLUPS "
+                f"{r['GBs']:.0f} GB/s")
     return out
 
 
@@ -331,6 +353,17 @@ def north_star_ratios(variants: list, adjoint: list, head: dict, copy_gbs: float
         for st in ("", "-fp16", "-bf16"):
             out[f"d2q9_2step{st.replace('-', '_')}_over_1step_{coll}"] = ratio(
                 leg(f"fused-2step{st}", coll), leg(f"fused{st}", coll))
+
+    def walled(name, variant):
+        for r in variants:
+            if r.get("workload", "").startswith(f"walled {name} ") and r["variant"] == variant:
+                return r["MLUPS"]
+        return None
+
+    for name in ("cavity", "channel"):
+        for st in ("", "-fp16"):
+            out[f"walled_{name}_2step{st.replace('-', '_')}_over_1step_bgk"] = ratio(
+                walled(name, f"fused-2step{st}"), walled(name, f"fused{st}"))
     return out
 
 
@@ -348,7 +381,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--quick", action="store_true", help="headline + copy ceiling only")
     ap.add_argument("--profile", nargs="?", const="d2q9", default=None,
-                    choices=["d2q9", "d2q9-2step", "d2q9-2step-fp16", "d3q19", "d3q19-fp16",
+                    choices=["d2q9", "d2q9-2step", "d2q9-2step-fp16", "cavity", "cavity-fp16",
+                             "cavity-2step", "cavity-2step-fp16", "d3q19", "d3q19-fp16",
                              "d3q19-aa", "d3q19-aa-fp16",
                              "geometry-d2q9", "geometry-d3q19", "copy", "planes", "sweep"],
                     help="only one step kernel and the copy kernel, for rocprofv3: the headline "
@@ -463,6 +497,13 @@ def main() -> int:
             s = B.make("hip", v, p, init="none")
             s.init_equilibrium(rho, u)
             nbytes = 9 * n * n * 4  # the bytes one step reads (and writes)
+        elif a.profile.startswith("cavity"):
+            # the lid-driven cavity n^2 (walls on both axes, lid 0.1), BGK tau 0.6, from rest
+            p = Problem((n, n), tau=0.6, collision="bgk", u_lid=0.1)
+            v = "fused" + a.profile[len("cavity"):]
+            s = B.make("hip", v, p, init="none")
+            s.init_equilibrium(None, None)
+            nbytes = 9 * n * n * 4
         elif a.profile.startswith("geometry"):
             p = cylinder_array_channel(n) if a.profile.endswith("d2q9") else sphere_array_box(256)
             s = B.make("hip", "fused-fp16", p, init="none")
@@ -543,6 +584,7 @@ def main() -> int:
             log(f"  D3Q19 {n3}^3 trt hip/{v}: {r['MLUPS']:.0f} MLUPS {r['GBs']:.0f} GB/s")
         del init3
         torch.cuda.empty_cache()
+        variants += walled_legs(n, proto)
         variants += geometry_legs(n, proto)
     for r in variants:
         r["pct_copy"] = 100.0 * r["GBs"] / cbw

@@ -106,18 +106,19 @@ def ladder(bench: dict, lattice: str, collision: str = "bgk",
     """One run per (backend, variant): the largest size it measured (ties: the faster run).
 
     Runs that carry a non-empty ``workload`` (obstacle channels and the like) are kept out
-    unless ``workload`` is True, in which case only they are taken."""
-    best: dict[tuple[str, str], dict] = {}
+    unless ``workload`` is True, in which case only they are taken, one run per (workload,
+    backend, variant)."""
+    best: dict[tuple[str, str, str], dict] = {}
     for r in bench.get("runs", []):
         if r.get("lattice") != lattice or r.get("collision") != collision:
             continue
         if bool(r.get("workload")) != workload:
             continue
-        k = rung_key(r)
+        k = (r.get("workload") or "",) + rung_key(r)
         cur = best.get(k)
         if cur is None or (_nodes(r), r["median"]) > (_nodes(cur), cur["median"]):
             best[k] = r
-    return [best[k] for k in sorted(best, key=_rank)]
+    return [best[k] for k in sorted(best, key=lambda k: (k[0],) + _rank(k[1:]))]
 
 
 def ceiling_mlups(bench: dict, bytes_per_update: float) -> float | None:

@@ -97,9 +97,9 @@ k_step_pair(const KArgs a, const typename Codec<ST>::T* __restrict__ A,
   // Every lane runs the same loads and stores, so that the compiler's count of outstanding
   // memory operations holds on every path (a branch around a load or store would make it
   // wait for the stores of step B before the next row's loads): lanes past the end of a
-  // partial tile repeat its last column (the same values to the same addresses), and every
-  // wave loads a halo column (lane 0 the left one, the others the right one), which only
-  // the first wave collides.
+  // partial tile repeat its last column (the same values to the same addresses), and the
+  // waves that load the halo columns (lane 0 the left one, the others the right one) do so
+  // in every row, though only the first wave collides them.
   const int lc = min(lane, w - 1);
   const int xo = x0 + lc, col = lc + 1;
   const int xh = lane == 0 ? (x0 - 1 + nx) % nx : (x0 + w) % nx;
@@ -125,8 +125,10 @@ k_step_pair(const KArgs a, const typename Codec<ST>::T* __restrict__ A,
   // step A, and in step B they read the rings of the rows behind the wall (-1 and ny), which
   // hold the bounced-back values (bounce(), below), so that step B has no wall test except
   // the per-lane x-wall one of MODE 2.
-  auto body = [&](auto mode_tag) {
+  auto body = [&](auto mode_tag, auto halo_tag) {
     constexpr int MODE = decltype(mode_tag)::value;
+    // HALO: this wave loads the halo columns (see dispatch below)
+    constexpr bool HALO = decltype(halo_tag)::value;
     // Step A's loads at (x, row p): slot i from (x - c_x, p - c_y), periodic, and the force
     // of the node.  With walls, a link whose source lies behind a wall reads the own node's
     // slot opp(i) instead (half-way bounce-back): a y-wall link for the whole row (slot,
@@ -213,7 +215,7 @@ k_step_pair(const KArgs a, const typename Codec<ST>::T* __restrict__ A,
     float FA[2], FH[2], Fm2[2], Fm1[2];  // forces of the own column at rows r, r - 2, r - 1
     auto issue = [&](int p) {
       issue1(xo, false, p, tA, eA, FA);
-      issue1(xh, true, p, tH, eH, FH);
+      if constexpr (HALO) issue1(xh, true, p, tH, eH, FH);
     };
     // MODE 2: a link that crosses an x wall (c_x > 0 at x = 0, c_x < 0 at x = nx - 1; also
     // at a corner, where the y-wall load holds the same value) takes the own node's slot.
@@ -232,15 +234,18 @@ k_step_pair(const KArgs a, const typename Codec<ST>::T* __restrict__ A,
     auto finish = [&](int p) {
       xwall(xo, false, tA, eA, eH);
       finish1(p, col, tA, FA);
-      xwall(xh, true, tH, eH, eH);
-      if constexpr (WALLS) {
-        // every wave takes its halo loads here (only the first wave collides them), so that
-        // none is still counted as outstanding when its registers are reused in the next
-        // row (which would make the compiler wait there for step B's stores too)
+      if constexpr (HALO) {
+        xwall(xh, true, tH, eH, eH);
+        if constexpr (WALLS) {
+          // every wave that loads the halo takes those loads here (only the first wave
+          // collides them), so that none is still counted as outstanding when its registers
+          // are reused in the next row (which would make the compiler wait there for step
+          // B's stores too)
 #pragma unroll
-        for (int i = 0; i < Q; ++i) asm volatile("" ::"v"(tH[i]));
+          for (int i = 0; i < Q; ++i) asm volatile("" ::"v"(tH[i]));
+        }
+        if (wave0) finish1(p, ch, tH, FH);
       }
-      if (wave0) finish1(p, ch, tH, FH);
       Fm2[0] = Fm1[0];
       Fm2[1] = Fm1[1];
       Fm1[0] = FA[0];
@@ -321,12 +326,21 @@ k_step_pair(const KArgs a, const typename Codec<ST>::T* __restrict__ A,
     }
     stepB(y1 - 1);
   };
+  // Only the first wave collides the halo columns, so in MODE 0 and 1 the other waves run a
+  // copy of the body without the halo loads.  The choice is per wave and made once, outside
+  // the row loop, so each copy keeps its loads and stores straight; both copies pass the
+  // same workgroup barriers.  MODE 2 keeps the halo loads on every wave: for nx = 1 the own
+  // column takes its x-wall slots from them.
+  using HaloOn = std::true_type;
+  using HaloOff = std::false_type;
   if (xt != 0 && x0 + w < nx - 1) {
-    body(std::integral_constant<int, 0>{});
+    if (wave0) body(std::integral_constant<int, 0>{}, HaloOn{});
+    else body(std::integral_constant<int, 0>{}, HaloOff{});
   } else if (WALLS && wlx) {
-    body(std::integral_constant<int, 2>{});
+    body(std::integral_constant<int, 2>{}, HaloOn{});
   } else {
-    body(std::integral_constant<int, 1>{});
+    if (wave0) body(std::integral_constant<int, 1>{}, HaloOn{});
+    else body(std::integral_constant<int, 1>{}, HaloOff{});
   }
 }
 
@@ -350,10 +364,12 @@ inline int pair_chunk_rows(int nxt, int ny, int slots) {
   return Ly < 1 ? 1 : Ly;
 }
 
-// workgroup slots of kernel k on the current device (cached per device; host threads may
-// race to fill an entry, with the same value)
-template <class K>
-int pair_slots(K k) {
+// workgroup slots of kernel KERNEL on the current device, cached per kernel and device
+// (each instantiation of this template has its own cache: kernels of one signature differ in
+// their registers and so in their occupancy; host threads may race to fill an entry, with
+// the same value)
+template <auto KERNEL>
+int pair_slots() {
   static std::atomic<int> cache[64] = {};
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 2048;
@@ -361,7 +377,7 @@ int pair_slots(K k) {
   if (known > 0) return known;
   int cus = 0, nb = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, kPairCols, 0) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, KERNEL, kPairCols, 0) != hipSuccess ||
       cus <= 0 || nb <= 0)
     return 2048;
   cache[dev].store(cus * nb, std::memory_order_relaxed);
@@ -373,7 +389,7 @@ int launch_step_pair_w(const KArgs& a, const void* src, void* dst, hipStream_t s
   using T = typename Codec<ST>::T;
   auto k = k_step_pair<COLL, ST, FM, WALLS>;
   const int nxt = (a.nx + kPairCols - 1) / kPairCols;
-  const int Ly = pair_chunk_rows(nxt, a.ny, pair_slots(k));
+  const int Ly = pair_chunk_rows(nxt, a.ny, pair_slots<k_step_pair<COLL, ST, FM, WALLS>>());
   const int nyc = (a.ny + Ly - 1) / Ly;
   const int64_t wgs = (int64_t)nxt * nyc;
   if (wgs > INT32_MAX) return PYLBM_ERANGE;
@@ -450,7 +466,7 @@ int launch_sweep_probe_t(const KArgs& a, const void* src, void* dst, hipStream_t
   using T = typename Codec<ST>::T;
   const int nxt = (a.nx + kPairCols - 1) / kPairCols;
   const int Ly =
-      pair_chunk_rows(nxt, a.ny, pair_slots(k_step_pair<PYLBM_BGK, ST, kForceNone, false>));
+      pair_chunk_rows(nxt, a.ny, pair_slots<k_step_pair<PYLBM_BGK, ST, kForceNone, false>>());
   const int nyc = (a.ny + Ly - 1) / Ly;
   const int64_t wgs = (int64_t)nxt * nyc;
   if (wgs > INT32_MAX) return PYLBM_ERANGE;

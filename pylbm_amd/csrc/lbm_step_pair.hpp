@@ -232,7 +232,9 @@ k_step_pair(const KArgs a, const typename Codec<ST>::T* __restrict__ A,
       }
     };
     auto finish = [&](int p) {
-      xwall(xo, false, tA, eA, eH);
+      // eH serves the own column only for nx = 1, where every wave loads the halo
+      if constexpr (HALO) xwall(xo, false, tA, eA, eH);
+      else xwall(xo, false, tA, eA, eA);
       finish1(p, col, tA, FA);
       if constexpr (HALO) {
         xwall(xh, true, tH, eH, eH);
@@ -326,18 +328,19 @@ k_step_pair(const KArgs a, const typename Codec<ST>::T* __restrict__ A,
     }
     stepB(y1 - 1);
   };
-  // Only the first wave collides the halo columns, so in MODE 0 and 1 the other waves run a
-  // copy of the body without the halo loads.  The choice is per wave and made once, outside
-  // the row loop, so each copy keeps its loads and stores straight; both copies pass the
-  // same workgroup barriers.  MODE 2 keeps the halo loads on every wave: for nx = 1 the own
-  // column takes its x-wall slots from them.
+  // Only the first wave collides the halo columns, so the other waves run a copy of the
+  // body without the halo loads.  The choice is per wave and made once, outside the row
+  // loop, so each copy keeps its loads and stores straight; both copies pass the same
+  // workgroup barriers.  With x walls and nx = 1 every wave loads the halo: the own column
+  // takes its x-wall slots from those loads (the halo column is the own node there).
   using HaloOn = std::true_type;
   using HaloOff = std::false_type;
   if (xt != 0 && x0 + w < nx - 1) {
     if (wave0) body(std::integral_constant<int, 0>{}, HaloOn{});
     else body(std::integral_constant<int, 0>{}, HaloOff{});
   } else if (WALLS && wlx) {
-    body(std::integral_constant<int, 2>{}, HaloOn{});
+    if (wave0 || nx == 1) body(std::integral_constant<int, 2>{}, HaloOn{});
+    else body(std::integral_constant<int, 2>{}, HaloOff{});
   } else {
     if (wave0) body(std::integral_constant<int, 1>{}, HaloOn{});
     else body(std::integral_constant<int, 1>{}, HaloOff{});
